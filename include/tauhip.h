@@ -1,0 +1,136 @@
+/*
+ * tauhip.h — C-ABI of the MI355X (gfx950) steady-state diffusion solve behind
+ * taufactor_amd.Solver / taufactor_amd.MultiPhaseSolver.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (0 = ok, <0 = library error, >0 = hipError_t) and never throws.
+ * The message of the last failure on the calling thread is tf_last_error().
+ * Device pointers are caller-owned (PyTorch-ROCm tensors); the library does
+ * not allocate device memory. `stream` is a hipStream_t passed as void*; all
+ * device work is enqueued asynchronously on it.
+ *
+ * Volume layout (shared by every call):
+ *   field  float32 [B][X+2][Y+2][Z+2], z fastest.  Padded exactly like the
+ *          reference's SORSolver.field: x ghosts hold 2*bc (-1 at x=0, +1 at
+ *          x=X+1), y/z ghosts hold 0.
+ *   codes  uint8   [B][X+2][Y+2][Z+2], same indexing as field.
+ *          kind TF_KIND_SOLVER:     code = number of conductive neighbours
+ *                                   (x ghost counts 2), 0 = factor inf
+ *                                   (non-conductive voxel, or conductive
+ *                                   voxel with no conductive neighbour).
+ *          kind TF_KIND_MULTIPHASE: code = phase class index into the
+ *                                   n_classes x n_classes face-conductance
+ *                                   table; x ghosts copy the adjacent voxel's
+ *                                   class, y/z ghosts hold class 0, which is
+ *                                   always a zero-diffusivity class.
+ *   Interior voxel (x,y,z) (0-based) has colour (x+y+z)&1; reference
+ *   iteration k updates colour k&1 (colour 0 first).
+ */
+#ifndef TAUHIP_H
+#define TAUHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TF_ABI_VERSION 1
+
+#define TF_KIND_SOLVER 0
+#define TF_KIND_MULTIPHASE 1
+
+/* Tile shape of the temporally blocked sweep (interior voxels per workgroup
+ * in y, z and x).  Zero fields select the built-in defaults. */
+typedef struct tf_tile {
+    int ty, tz, tx;
+} tf_tile;
+
+/* ABI version of the loaded library (TF_ABI_VERSION). No reference
+ * counterpart: lets the Python loader refuse a stale build. */
+int tf_abi_version(void);
+
+/* Thread-local message for the last non-zero status. */
+const char* tf_last_error(void);
+
+/* Set-up of a two-phase volume.  Replaces SORSolver.__init__ field/prefactor
+ * construction: ThroughTransportSolver.init_field (taufactor/taufactor.py:282-291),
+ * SORSolver._pad (:227-238), Solver.init_conductive_neighbours (:402-410)
+ * and vol_x = mean(mask,(2,3)) (:42).
+ *   mask    device uint8 [B][X][Y][Z], 1 = conductive, 0 = not
+ *   xinit   device float [X], the linear initial profile (linspace of
+ *           -0.5+1/(2X) .. 0.5-1/(2X) in float32)
+ *   codes   device uint8 padded, written (ghosts 0)
+ *   field_a, field_b  device float padded, both written with the initial field
+ *   layer_count device uint32 [B][X], written with the number of
+ *           conductive voxels per x layer.                               */
+int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t* codes,
+                    float* field_a, float* field_b, uint32_t* layer_count,
+                    int B, int X, int Y, int Z, void* stream);
+
+/* Set-up of a labelled volume for the face-conductance stencil.  Replaces
+ * MultiPhaseSolver.return_mask / init_conductive_neighbours
+ * (taufactor/taufactor.py:571-604) and the shared field set-up above.
+ *   cls      device uint8 [B][X][Y][Z], class index per voxel (< n_classes)
+ *   cls_conductive device uint8 [n_classes], 1 if the class diffusivity > 0
+ *   Other arguments as tf_setup_solver.                                    */
+int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
+                        int n_classes, const float* xinit, uint8_t* codes,
+                        float* field_a, float* field_b, uint32_t* layer_count,
+                        int B, int X, int Y, int Z, void* stream);
+
+/* n_half over-relaxed red-black half-sweeps (reference iterations), the
+ * first one on colour first_colour, fused in one HBM pass: reads src,
+ * writes every interior voxel of dst (ghosts of dst are left untouched and
+ * must already hold the ghost values).  src and dst must not alias.
+ * Replaces the loop body of SORSolver.solve (taufactor/taufactor.py:175-182)
+ * with SORSolver.sum_weighted_neighbours (:95-103) or
+ * MultiPhaseSolver.sum_weighted_neighbours (:606-613).  Arithmetic is the
+ * reference's float32 operation order, without contraction, so the result
+ * equals n_half reference iterations value for value.
+ *   hm_lut   device float [n_classes*n_classes] face conductances
+ *            (TF_KIND_MULTIPHASE), ignored for TF_KIND_SOLVER
+ *   omega    the over-relaxation factor rounded to float32
+ *   n_half   1..TF_MAX_FUSED_HALF_SWEEPS
+ *   tile     NULL for the default tile shape                              */
+#define TF_MAX_FUSED_HALF_SWEEPS 8
+int tf_sweep(const float* src, float* dst, const uint8_t* codes,
+             const float* hm_lut, int n_classes, int kind,
+             int B, int X, int Y, int Z, float omega, int first_colour,
+             int n_half, const tf_tile* tile, void* stream);
+
+/* One half-sweep of colour `colour` in place, one thread per voxel.  The
+ * unblocked form of tf_sweep, kept as the simplest statement of the update
+ * for tests; same arithmetic.                                            */
+int tf_sweep_half_inplace(float* field, const uint8_t* codes,
+                          const float* hm_lut, int n_classes, int kind,
+                          int B, int X, int Y, int Z, float omega, int colour,
+                          void* stream);
+
+/* Per-layer float64 sums for the convergence check.  Replaces
+ * Solver.vertical_flux (taufactor/taufactor.py:412-419) /
+ * MultiPhaseSolver.vertical_flux (:615-620) and the two torch.mean(...,(2,3))
+ * reductions of ThroughTransportSolver.compute_metrics (:296, :307).
+ *   flux_sum device double [B][X-1]: sum over (y,z) of the float32 face flux
+ *            (c[x+1]-c[x], zeroed when either voxel has code 0, for
+ *            TF_KIND_SOLVER; hm*(c[x+1]-c[x]) for TF_KIND_MULTIPHASE)
+ *   conc_sum device double [B][X]:   sum over (y,z) of c
+ * The summation order is fixed, so repeated calls give identical bits.    */
+int tf_layer_sums(const float* field, const uint8_t* codes,
+                  const float* hm_lut, int n_classes, int kind,
+                  int B, int X, int Y, int Z,
+                  double* flux_sum, double* conc_sum, void* stream);
+
+/* Host-side test: does any 6-connected cluster of `mask` (host uint8
+ * [X][Y][Z]) touch both x faces?  Returns 1 or 0, or <0 on error.
+ * Replaces the spanning test of extract_through_feature
+ * (taufactor/metrics/connectivity.py:138-213, find_spanning_labels :83-115)
+ * as called from compute_metrics (taufactor/taufactor.py:318-327).        */
+int tf_spans_x(const uint8_t* mask, int X, int Y, int Z);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAUHIP_H */

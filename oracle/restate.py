@@ -1,0 +1,283 @@
+"""Oracle: CPU restatement of the reference's Solver / MultiPhaseSolver solve.
+
+TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module; it is the checker of
+the HIP path, never part of it.
+
+It restates, op for op in float32 torch on the CPU, what the reference does:
+  set-up      taufactor/taufactor.py:24-67 (SORSolver.__init__), 218-253
+              (_init_chequerboard, _pad, _crop, _sum_by_rolling), 282-291
+              (init_field), 380-410 (Solver), 535-604 (MultiPhaseSolver)
+  iteration   taufactor/taufactor.py:95-103, 174-182, 606-613
+  check       taufactor/taufactor.py:109-153 (check_convergence),
+              293-331 (compute_metrics), 412-419 / 615-620 (vertical_flux)
+  percolation taufactor/metrics/connectivity.py:83-115, 138-213
+Because torch's elementwise float32 operations are IEEE-rounded one by one,
+the field after k iterations here is the reference's field after k
+iterations on the CPU.
+
+The per-layer means of the check are computed either in float32 with
+torch.mean, exactly as the reference does (reduce="f32"), or as float64 sums
+of the same float32 face values (reduce="f64"), the form the HIP path uses.
+
+Pinned by (tests/test_oracle_*.py): the reference's own known answers
+(tests/test_taufactor.py:12-247), the stored outputs of
+docs/notebooks/12-solver-benchmark.ipynb (tau and iteration counts for fcc
+N=32..128 and the multiphase zigzag), an FP64 sparse direct solve of the
+same discrete operator (oracle/direct.py) and closed forms.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.ndimage
+import torch
+
+TOP_BC, BOT_BC = -0.5, 0.5
+
+
+def expand_to_4d(img: np.ndarray) -> np.ndarray:
+    """taufactor/taufactor.py:194-204"""
+    if img.ndim == 2:
+        img = img[..., None]
+    if img.ndim == 3:
+        img = img[None, ...]
+    assert img.ndim == 4
+    return img
+
+
+def default_omega(nx: int) -> float:
+    """taufactor/taufactor.py:36-37"""
+    return 2 - math.pi / (1.5 * nx)
+
+
+def pad(t: torch.Tensor, vals) -> torch.Tensor:
+    """Pad the three spatial dims by one voxel and set the ghost values
+    (x faces, then y faces, then z faces; taufactor/taufactor.py:227-238)."""
+    vals = list(vals) + [0] * (6 - len(vals))
+    out = torch.nn.functional.pad(t, [1, 1, 1, 1, 1, 1], "constant")
+    out[:, 0], out[:, -1] = vals[0], vals[1]
+    out[:, :, 0], out[:, :, -1] = vals[2], vals[3]
+    out[:, :, :, 0], out[:, :, :, -1] = vals[4], vals[5]
+    return out
+
+
+def chequerboards(nx, ny, nz, omega):
+    """[omega*cb, omega*(1-cb)] in float32, cb = 1 where (i+j+k) is even
+    (taufactor/taufactor.py:218-225)."""
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    cb = ((i + j + k) % 2 == 0).astype(np.float64)
+    return [torch.tensor(omega * cb, dtype=torch.float32), torch.tensor(omega * (1 - cb), dtype=torch.float32)]
+
+
+def init_field(mask: torch.Tensor) -> torch.Tensor:
+    """mask * linear profile, padded with 2*bc on the x faces
+    (taufactor/taufactor.py:282-291)."""
+    B, X, Y, Z = mask.shape
+    sh = 1 / (2 * X)
+    vec = torch.linspace(TOP_BC + sh, BOT_BC - sh, X, dtype=torch.float32)
+    vec = vec.view(1, X, 1, 1).repeat(B, 1, Y, Z)
+    return pad(mask * vec, [2 * TOP_BC, 2 * BOT_BC])
+
+
+def spans_x(mask3d: np.ndarray) -> bool:
+    """Is any 6-connected cluster present on both x faces?
+    (extract_through_feature(...)[1] != 0; taufactor/metrics/connectivity.py:138-213)"""
+    if not mask3d.any():
+        return False
+    lab, _ = scipy.ndimage.label(mask3d, structure=scipy.ndimage.generate_binary_structure(3, 1))
+    first = set(np.unique(lab[0])) - {0}
+    last = set(np.unique(lab[-1])) - {0}
+    return len(first & last) > 0
+
+
+class OracleSolve:
+    """Common loop; subclasses provide factor / weighted sum / flux."""
+
+    def __init__(self, img, omega=None, reduce="f64"):
+        self.cpu_img = expand_to_4d(np.asarray(img))
+        self.B, self.Nx, self.Ny, self.Nz = self.cpu_img.shape
+        self.omega = default_omega(self.Nx) if omega is None else omega
+        self.reduce = reduce
+        torch_img = torch.tensor(self.cpu_img, dtype=torch.float32)
+        mask = self.mask(torch_img)
+        self.vol_x = torch.mean(mask, (2, 3)).numpy()
+        self.field = init_field(mask)
+        self.factor = self.init_factor(torch_img, mask)
+        self.cb = chequerboards(self.Nx, self.Ny, self.Nz, self.omega)
+        self.iter = 0
+        self.old_tau = 0
+        self.converged = False
+        self.tau = None
+        self.tau_x = None
+        self.D_eff = None
+        self.n_checks = 0
+
+    # one reference iteration (taufactor/taufactor.py:175-182)
+    def step(self):
+        inc = self.weighted_sum()
+        inc /= self.factor
+        inc -= self.field[:, 1:-1, 1:-1, 1:-1]
+        inc *= self.cb[self.iter % 2]
+        self.field[:, 1:-1, 1:-1, 1:-1] += inc
+        self.iter += 1
+
+    def run(self, n):
+        with torch.no_grad():
+            for _ in range(n):
+                self.step()
+        return self
+
+    def _layer_mean(self, t: torch.Tensor) -> np.ndarray:
+        if self.reduce == "f32":
+            return torch.mean(t, (2, 3)).numpy()
+        return (t.to(torch.float64).sum((2, 3)) / (t.shape[2] * t.shape[3])).numpy()
+
+    def compute_metrics(self):
+        """taufactor/taufactor.py:293-331"""
+        vf = self.vertical_flux()
+        flux_1d = self._layer_mean(vf)
+        fl_max = np.max(flux_1d, axis=1)
+        fl_min = np.min(flux_1d, axis=1)
+        mean_fl = np.mean(flux_1d, axis=1)
+        rel = np.divide(fl_max - fl_min, fl_max, out=np.full_like(fl_max, np.nan), where=fl_max != 0)
+        D_rel = mean_fl * self.Nx / abs(TOP_BC - BOT_BC)
+        tau = np.divide(self.D_mean, D_rel, out=np.full_like(D_rel, np.nan), where=D_rel != 0)
+        c_x = self._layer_mean(self.field[:, 1:-1, 1:-1, 1:-1])
+        vol_x = self.vol_x.astype(c_x.dtype)
+        c_x = np.divide(c_x, vol_x, out=np.zeros_like(vol_x), where=vol_x != 0)
+        self.c_x = c_x
+        dfc = c_x[:, 1:] - c_x[:, :-1]
+        dfc[vol_x[:, 1:] == 0] = 0
+        dfc[vol_x[:, :-1] == 0] = 0
+        eps = 0.5 * (vol_x[:, :-1] + vol_x[:, 1:])
+        self.tau_x = np.divide(eps * dfc, flux_1d, out=np.full_like(dfc, np.nan), where=flux_1d != 0)
+        for b in range(self.B):
+            if fl_min[b] == 0 or fl_max[b] == 0 or mean_fl[b] == 0:
+                if not spans_x(np.isin(self.cpu_img[b], self.conductive_labels)):
+                    rel[b] = 0
+                    D_rel[b] = 0
+                    tau[b] = 0
+                    self.tau_x[b, :] = 0
+        rel[np.isnan(mean_fl)] = 0
+        self.flux_1d = flux_1d
+        self.D_rel = D_rel
+        self.D_eff = self.D_0 * D_rel
+        return tau, rel
+
+    def check_convergence(self, conv_crit):
+        """taufactor/taufactor.py:109-153"""
+        self.n_checks += 1
+        self.tau, rel = self.compute_metrics()
+        self.rel_err = rel
+        if not np.all(rel < conv_crit):
+            self.old_tau = self.tau
+            return False
+        if not np.max(np.abs(self.tau - self.old_tau)) < 2e-3:
+            self.old_tau = self.tau
+            return False
+        self.tau[self.tau == 0] = np.inf
+        return True
+
+    def solve(self, iter_limit=10000, conv_crit=1e-2):
+        """taufactor/taufactor.py:156-191 without the printing"""
+        with torch.no_grad():
+            while not self.converged and self.iter < iter_limit:
+                self.step()
+                if self.iter % 100 == 0:
+                    self.converged = self.check_convergence(conv_crit)
+        return self
+
+
+class OracleSolver(OracleSolve):
+    """taufactor/taufactor.py:380-419"""
+
+    conductive_labels = [1]
+
+    def __init__(self, img, omega=None, D_0=1, reduce="f64"):
+        super().__init__(img, omega, reduce)
+        self.D_0 = D_0
+        self.D_mean = np.mean(self.vol_x, axis=1)
+
+    def mask(self, torch_img):
+        return torch_img
+
+    def init_factor(self, torch_img, mask):
+        """count of conductive neighbours, x ghosts counting 2 (taufactor/taufactor.py:402-410)"""
+        p = pad(mask, [2, 2])
+        nn = (p[:, 2:, 1:-1, 1:-1] + p[:, :-2, 1:-1, 1:-1] + p[:, 1:-1, 2:, 1:-1] +
+              p[:, 1:-1, :-2, 1:-1] + p[:, 1:-1, 1:-1, 2:] + p[:, 1:-1, 1:-1, :-2])
+        nn[mask == 0] = torch.inf
+        nn[nn == 0] = torch.inf
+        return nn
+
+    def weighted_sum(self):
+        f = self.field
+        return (f[:, 2:, 1:-1, 1:-1] + f[:, :-2, 1:-1, 1:-1] + f[:, 1:-1, 2:, 1:-1] +
+                f[:, 1:-1, :-2, 1:-1] + f[:, 1:-1, 1:-1, 2:] + f[:, 1:-1, 1:-1, :-2])
+
+    def vertical_flux(self):
+        f = self.field
+        vf = f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1]
+        vf[self.factor[:, 0:-1] > 8] = 0
+        vf[self.factor[:, 1:] > 8] = 0
+        return vf
+
+
+def harmonic_mean(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """2ab/(a+b), 0 where a+b == 0 (taufactor/taufactor.py:577-583)"""
+    denom = a + b
+    hm = torch.zeros_like(denom)
+    valid = denom > 0
+    hm[valid] = 2 * a[valid] * b[valid] / denom[valid]
+    return hm
+
+
+class OracleMultiPhaseSolver(OracleSolve):
+    """taufactor/taufactor.py:535-620"""
+
+    def __init__(self, img, diffusivities=None, D_scaling=1, omega=None, reduce="f64"):
+        Ds = dict({0: 0, 1: 1} if diffusivities is None else diffusivities)
+        for lbl in np.unique(img):
+            if lbl not in Ds:
+                Ds[int(lbl)] = 0.0
+        self.Ds = Ds
+        self.conductive_labels = [lbl for lbl, d in Ds.items() if d > 0]
+        super().__init__(img, omega, reduce)
+        self.VF = {int(p): np.mean(self.cpu_img == p, axis=(1, 2, 3)) for p in np.unique(self.cpu_img)}
+        self.D_0 = D_scaling
+        self.D_mean = np.sum([self.VF[z] * self.Ds.get(z, 0.0) for z in self.VF], axis=0)
+
+    def mask(self, torch_img):
+        if not self.conductive_labels:
+            return torch.zeros_like(torch_img)
+        return torch.isin(torch_img, torch.tensor(self.conductive_labels, dtype=torch.float32)).to(torch.float32)
+
+    def init_factor(self, torch_img, mask):
+        """face conductances and their sum (taufactor/taufactor.py:585-604)"""
+        dmap = torch.zeros_like(torch_img)
+        for phase, d in self.Ds.items():
+            dmap[torch_img == phase] = d
+        dmap = pad(dmap, [0, 0])
+        dmap[:, 0] = dmap[:, 1]
+        dmap[:, -1] = dmap[:, -2]
+        self.D_x = harmonic_mean(dmap[:, :-1, 1:-1, 1:-1], dmap[:, 1:, 1:-1, 1:-1])
+        self.D_y = harmonic_mean(dmap[:, 1:-1, :-1, 1:-1], dmap[:, 1:-1, 1:, 1:-1])
+        self.D_z = harmonic_mean(dmap[:, 1:-1, 1:-1, :-1], dmap[:, 1:-1, 1:-1, 1:])
+        factor = (self.D_x[:, :-1] + self.D_x[:, 1:] + self.D_y[:, :, :-1] + self.D_y[:, :, 1:] +
+                  self.D_z[:, :, :, :-1] + self.D_z[:, :, :, 1:])
+        factor[:, 0] += self.D_x[:, 0]
+        factor[:, -1] += self.D_x[:, -1]
+        factor[factor == 0] = torch.inf
+        return factor
+
+    def weighted_sum(self):
+        f = self.field
+        return (f[:, 2:, 1:-1, 1:-1] * self.D_x[:, 1:] + f[:, :-2, 1:-1, 1:-1] * self.D_x[:, :-1] +
+                f[:, 1:-1, 2:, 1:-1] * self.D_y[:, :, 1:] + f[:, 1:-1, :-2, 1:-1] * self.D_y[:, :, :-1] +
+                f[:, 1:-1, 1:-1, 2:] * self.D_z[:, :, :, 1:] + f[:, 1:-1, 1:-1, :-2] * self.D_z[:, :, :, :-1])
+
+    def vertical_flux(self):
+        f = self.field
+        return self.D_x[:, 1:-1] * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])

@@ -1,0 +1,11 @@
+"""taufactor_amd — MI355X-native steady-state diffusion solve with the
+TauFactor solver API (``Solver``, ``MultiPhaseSolver``).
+
+The iteration runs in hand-written gfx950 HIP kernels (libtauhip.so, C-ABI in
+include/tauhip.h) loaded through ctypes; PyTorch-ROCm holds the device
+buffers.  There is no CPU code path.
+"""
+from .solver import MultiPhaseSolver, SORSolver, Solver, ThroughTransportSolver
+
+__all__ = ["Solver", "MultiPhaseSolver", "SORSolver", "ThroughTransportSolver"]
+__version__ = "0.1.0"

@@ -1,0 +1,93 @@
+"""ctypes binding of libtauhip.so (the C-ABI declared in include/tauhip.h).
+
+The library is loaded once, from the package directory.  There is no fallback:
+if it is missing or stale, every solver call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtauhip.so")
+ABI_VERSION = 1
+
+KIND_SOLVER = 0
+KIND_MULTIPHASE = 1
+MAX_FUSED_HALF_SWEEPS = 8
+
+_c_int = ctypes.c_int
+_c_float = ctypes.c_float
+_vp = ctypes.c_void_p
+
+
+class Tile(ctypes.Structure):
+    _fields_ = [("ty", ctypes.c_int), ("tz", ctypes.c_int), ("tx", ctypes.c_int)]
+
+
+# name -> (restype, argtypes); the exported surface of include/tauhip.h
+SIGNATURES = {
+    "tf_abi_version": (_c_int, []),
+    "tf_last_error": (ctypes.c_char_p, []),
+    "tf_setup_solver": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_setup_multiphase": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp,
+                                     _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_sweep": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                          _c_float, _c_int, _c_int, ctypes.POINTER(Tile), _vp]),
+    "tf_sweep_half_inplace": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                       _c_int, _c_float, _c_int, _vp]),
+    "tf_layer_sums": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                               _vp, _vp, _vp]),
+    "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class TauHipError(RuntimeError):
+    """A non-zero status from libtauhip."""
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libtauhip.so and declare its signatures (cached)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise TauHipError(
+                f"{path} is missing: build it with `python -m taufactor_amd.csrc.build` "
+                "(or __graft_entry__.build()); taufactor_amd has no non-HIP code path")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        ver = lib.tf_abi_version()
+        if ver != ABI_VERSION:
+            raise TauHipError(f"{path} has ABI version {ver}, expected {ABI_VERSION}; rebuild it")
+        _lib = lib
+        return lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = load().tf_last_error()
+        msg = msg.decode(errors="replace") if msg else ""
+        raise TauHipError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a contiguous torch tensor / numpy array."""
+    if hasattr(t, "data_ptr"):
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+def stream_handle(device) -> int:
+    import torch
+    return torch.cuda.current_stream(device).cuda_stream

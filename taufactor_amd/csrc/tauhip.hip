@@ -1,0 +1,721 @@
+// tauhip.hip — gfx950 kernels and C-ABI for the steady-state diffusion solve
+// behind taufactor_amd.Solver / taufactor_amd.MultiPhaseSolver.
+//
+// Built with -ffp-contract=off: every float32 operation of the update is
+// rounded on its own, in the order the reference's eager torch ops use
+// (taufactor/taufactor.py:97-102, 176-181, 606-613), so iterates match the
+// reference value for value.  Fused multiply-adds appear only where written
+// explicitly (the exact-division sequence in div_by_count).
+//
+// Layout and conventions are documented in include/tauhip.h.
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "tauhip.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+static int hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    return fail((int)e, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define TF_LAUNCH_CHECK(what)                                   \
+    do {                                                        \
+        hipError_t e_ = hipGetLastError();                      \
+        if (e_ != hipSuccess) return hip_status(e_, what);      \
+    } while (0)
+
+extern "C" int tf_abi_version(void) { return TF_ABI_VERSION; }
+extern "C" const char* tf_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// geometry of one padded volume
+// ---------------------------------------------------------------------------
+struct Vol {
+    int X, Y, Z;     // interior extents
+    int sY, sX;      // padded strides: sY = Z+2, sX = (Y+2)(Z+2)
+    long long sB;    // padded volume size (X+2)(Y+2)(Z+2)
+};
+
+static Vol make_vol(int X, int Y, int Z) {
+    Vol v;
+    v.X = X; v.Y = Y; v.Z = Z;
+    v.sY = Z + 2;
+    v.sX = (Y + 2) * (Z + 2);
+    v.sB = (long long)(X + 2) * v.sX;
+    return v;
+}
+
+static int check_dims(int B, int X, int Y, int Z) {
+    if (B < 1 || X < 1 || Y < 1 || Z < 1)
+        return fail(-1, "invalid volume shape B=%d X=%d Y=%d Z=%d", B, X, Y, Z);
+    // offsets inside one padded volume are 32-bit
+    long long n = (long long)(X + 2) * (Y + 2) * (Z + 2);
+    if (n >= (1LL << 31))
+        return fail(-1, "padded volume of %lld voxels exceeds 2^31", n);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// exact float32 division by a conductive-neighbour count
+// ---------------------------------------------------------------------------
+// q = s / d for d in {inf, 1..8}, correctly rounded (the value IEEE division
+// gives).  One reciprocal product plus one residual correction with the
+// correctly rounded reciprocal r = RN(1/d); numerators below 2^-100 (where
+// the correction can be off for d = 6, and where the sign of zero matters)
+// take the IEEE division.  Exhaustively checked against s/d over all finite
+// float32 s for each d (tests/test_division.py samples that check).
+__device__ __forceinline__ float div_by_count(float s, float r, float d, int code) {
+    float q = s * r;
+    if (code != 0) {
+        float e = __builtin_fmaf(-q, d, s);
+        q = __builtin_fmaf(e, r, q);
+    }
+    if (__builtin_expect(__builtin_fabsf(s) < 0x1p-100f, 0)) q = s / d;
+    return q;
+}
+
+// table of (1/d, d) per Solver code; code 0 means factor = inf
+__constant__ float c_rcp[9] = {0.0f, 1.0f, 0.5f, 1.0f / 3.0f, 0.25f,
+                               0.2f, 1.0f / 6.0f, 1.0f / 7.0f, 0.125f};
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+// ---------------------------------------------------------------------------
+// One thread per padded voxel.  Writes codes and both field buffers.
+// mode 0: mask is 0/1 conductive flag, code = neighbour count
+// mode 1: mask is a class index, code = class (x ghosts copy the adjacent
+//         voxel's class, y/z ghosts class 0), conductive = cls_cond[class]
+template <int MODE>
+__global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
+                        const float* __restrict__ xinit, uint8_t* __restrict__ codes,
+                        float* __restrict__ fa, float* __restrict__ fb, Vol v, int B) {
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)B * v.sB;
+    if (gid >= total) return;
+    int b = (int)(gid / v.sB);
+    int r = (int)(gid - (long long)b * v.sB);
+    int xp = r / v.sX;
+    int r2 = r - xp * v.sX;
+    int yp = r2 / v.sY;
+    int zp = r2 - yp * v.sY;
+    const uint8_t* M = mask + (size_t)b * v.X * v.Y * v.Z;
+    auto m_at = [&](int x, int y, int z) -> int {  // interior 0-based
+        return (int)M[((size_t)x * v.Y + y) * v.Z + z];
+    };
+    bool yz_in = (yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z);
+    bool in = yz_in && xp >= 1 && xp <= v.X;
+    float f;
+    uint8_t code = 0;
+    if (in) {
+        int x = xp - 1, y = yp - 1, z = zp - 1;
+        int m = m_at(x, y, z);
+        int cond = MODE == 0 ? (m != 0) : (int)cls_cond[m];
+        // mask * linspace in float32 (taufactor/taufactor.py:291)
+        f = (cond ? 1.0f : 0.0f) * xinit[x];
+        if (MODE == 0) {
+            if (cond) {
+                // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410)
+                int nn = 0;
+                nn += (x + 1 < v.X) ? (m_at(x + 1, y, z) != 0) : 2;
+                nn += (x >= 1) ? (m_at(x - 1, y, z) != 0) : 2;
+                nn += (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
+                nn += (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
+                nn += (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
+                nn += (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
+                code = (uint8_t)nn;  // 0 stays 0 = inf
+            }
+        } else {
+            code = (uint8_t)m;
+        }
+    } else {
+        // ghost: x planes hold 2*bc on their y/z interior, everything else 0
+        // (SORSolver._pad, taufactor/taufactor.py:234-237)
+        f = 0.0f;
+        if (yz_in && xp == 0) f = -1.0f;
+        if (yz_in && xp == v.X + 1) f = 1.0f;
+        if (MODE == 1 && yz_in && (xp == 0 || xp == v.X + 1)) {
+            // diff_map[:,0] = diff_map[:,1] (taufactor/taufactor.py:591-592)
+            int x = xp == 0 ? 0 : v.X - 1;
+            code = (uint8_t)m_at(x, yp - 1, zp - 1);
+        }
+    }
+    size_t o = (size_t)b * v.sB + r;
+    codes[o] = code;
+    fa[o] = f;
+    fb[o] = f;
+}
+
+// conductive voxels per x layer; one block per (x, b)
+template <int MODE>
+__global__ void k_layer_count(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
+                              uint32_t* __restrict__ cnt, Vol v) {
+    int x = blockIdx.x, b = blockIdx.y;
+    const uint8_t* M = mask + ((size_t)b * v.X + x) * v.Y * v.Z;
+    int n = v.Y * v.Z;
+    uint32_t acc = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int m = M[i];
+        acc += MODE == 0 ? (m != 0) : cls_cond[m];
+    }
+    __shared__ uint32_t red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = red[0];
+}
+
+template <int MODE>
+static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const float* xinit,
+                        uint8_t* codes, float* fa, float* fb, uint32_t* cnt,
+                        int B, int X, int Y, int Z, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!mask || !xinit || !codes || !fa || !fb || !cnt || (MODE == 1 && !cls_cond))
+        return fail(-1, "null pointer argument");
+    if (fa == fb) return fail(-1, "field_a and field_b must be distinct buffers");
+    Vol v = make_vol(X, Y, Z);
+    hipStream_t st = (hipStream_t)stream;
+    long long total = (long long)B * v.sB;
+    int threads = 256;
+    long long blocks = (total + threads - 1) / threads;
+    hipLaunchKernelGGL(k_setup<MODE>, dim3((unsigned)blocks), dim3(threads), 0, st,
+                       mask, cls_cond, xinit, codes, fa, fb, v, B);
+    TF_LAUNCH_CHECK("k_setup");
+    hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, B), dim3(256), 0, st, mask, cls_cond, cnt, v);
+    TF_LAUNCH_CHECK("k_layer_count");
+    return 0;
+}
+
+extern "C" int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t* codes,
+                               float* field_a, float* field_b, uint32_t* layer_count,
+                               int B, int X, int Y, int Z, void* stream) {
+    return setup_common<0>(mask, nullptr, xinit, codes, field_a, field_b, layer_count,
+                           B, X, Y, Z, stream);
+}
+
+extern "C" int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
+                                   int n_classes, const float* xinit, uint8_t* codes,
+                                   float* field_a, float* field_b, uint32_t* layer_count,
+                                   int B, int X, int Y, int Z, void* stream) {
+    if (n_classes < 1 || n_classes > 256) return fail(-1, "n_classes=%d out of range", n_classes);
+    return setup_common<1>(cls, cls_conductive, xinit, codes, field_a, field_b, layer_count,
+                           B, X, Y, Z, stream);
+}
+
+// ---------------------------------------------------------------------------
+// the update of one voxel (shared by both sweep kernels)
+// ---------------------------------------------------------------------------
+// Solver (taufactor/taufactor.py:97-102, 177-181):
+//   s = ((((c[x+1]+c[x-1])+c[y+1])+c[y-1])+c[z+1])+c[z-1]
+//   s = s / nn ; s = s - c ; s = s * w ; c = c + s
+// MultiPhase (taufactor/taufactor.py:598-613):
+//   s = c[x+1]*gx+ + c[x-1]*gx- + c[y+1]*gy+ + c[y-1]*gy- + c[z+1]*gz+ + c[z-1]*gz-
+//   f = gx- + gx+ + gy- + gy+ + gz- + gz+ ; f += gx- at x=0 ; f += gx+ at x=X-1 ;
+//   f = inf if f == 0 ;  s = s / f ; s = s - c ; s = s * w ; c = c + s
+struct Nbr {
+    float c, xp, xm, yp, ym, zp, zm;
+};
+
+__device__ __forceinline__ float update_solver(const Nbr& n, int code, float w) {
+    float s = n.xp + n.xm;
+    s = s + n.yp;
+    s = s + n.ym;
+    s = s + n.zp;
+    s = s + n.zm;
+    float r = c_rcp[code];
+    float d = code ? (float)code : __builtin_inff();
+    s = div_by_count(s, r, d, code);
+    s = s - n.c;
+    s = s * w;
+    return n.c + s;
+}
+
+__device__ __forceinline__ float update_multiphase(const Nbr& n, float gxp, float gxm, float gyp,
+                                                   float gym, float gzp, float gzm, bool xfirst,
+                                                   bool xlast, float w) {
+    float s = n.xp * gxp + n.xm * gxm;
+    s = s + n.yp * gyp;
+    s = s + n.ym * gym;
+    s = s + n.zp * gzp;
+    s = s + n.zm * gzm;
+    float f = gxm + gxp;
+    f = f + gym;
+    f = f + gyp;
+    f = f + gzm;
+    f = f + gzp;
+    if (xfirst) f = f + gxm;
+    if (xlast) f = f + gxp;
+    if (f == 0.0f) f = __builtin_inff();
+    s = s / f;
+    s = s - n.c;
+    s = s * w;
+    return n.c + s;
+}
+
+// ---------------------------------------------------------------------------
+// unblocked in-place half-sweep: one thread per interior voxel
+// ---------------------------------------------------------------------------
+template <int KIND>
+__global__ void k_half_inplace(float* __restrict__ F, const uint8_t* __restrict__ C,
+                               const float* __restrict__ hm, int ncls, Vol v, float w, int colour) {
+    int z = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int bx = blockIdx.z;
+    int b = bx / v.X, x = bx - b * v.X;
+    if (z >= v.Z) return;
+    if (((x + y + z) & 1) != colour) return;
+    float* V = F + (size_t)b * v.sB;
+    const uint8_t* Cv = C + (size_t)b * v.sB;
+    int o = (x + 1) * v.sX + (y + 1) * v.sY + (z + 1);
+    Nbr n;
+    n.c = V[o];
+    n.xp = V[o + v.sX];
+    n.xm = V[o - v.sX];
+    n.yp = V[o + v.sY];
+    n.ym = V[o - v.sY];
+    n.zp = V[o + 1];
+    n.zm = V[o - 1];
+    float out;
+    if (KIND == TF_KIND_SOLVER) {
+        out = update_solver(n, Cv[o], w);
+    } else {
+        const float* row = hm + Cv[o] * ncls;
+        out = update_multiphase(n, row[Cv[o + v.sX]], row[Cv[o - v.sX]], row[Cv[o + v.sY]],
+                                row[Cv[o - v.sY]], row[Cv[o + 1]], row[Cv[o - 1]], x == 0,
+                                x == v.X - 1, w);
+    }
+    V[o] = out;
+}
+
+extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const float* hm_lut,
+                                     int n_classes, int kind, int B, int X, int Y, int Z,
+                                     float omega, int colour, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!field || !codes) return fail(-1, "null pointer argument");
+    if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 256))
+        return fail(-1, "multiphase sweep needs hm_lut and 1..256 classes");
+    if (colour != 0 && colour != 1) return fail(-1, "colour must be 0 or 1");
+    if ((long long)B * X > 65535 || Y > 65535) return fail(-1, "grid too large for unblocked sweep");
+    Vol v = make_vol(X, Y, Z);
+    dim3 grid((Z + 255) / 256, Y, B * X);
+    hipStream_t st = (hipStream_t)stream;
+    if (kind == TF_KIND_SOLVER)
+        hipLaunchKernelGGL(k_half_inplace<TF_KIND_SOLVER>, grid, dim3(256), 0, st, field, codes,
+                           hm_lut, n_classes, v, omega, colour);
+    else if (kind == TF_KIND_MULTIPHASE)
+        hipLaunchKernelGGL(k_half_inplace<TF_KIND_MULTIPHASE>, grid, dim3(256), 0, st, field,
+                           codes, hm_lut, n_classes, v, omega, colour);
+    else
+        return fail(-1, "unknown kind %d", kind);
+    TF_LAUNCH_CHECK("k_half_inplace");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// temporally blocked sweep: H half-sweeps per HBM pass
+// ---------------------------------------------------------------------------
+// A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
+// coordinates) and streams it plane by plane along x through a ring of H+2
+// planes in LDS, each plane extended by H voxels in y and z.  Half-sweep h
+// (h = 0..H-1, colour first_colour^(h&1)) runs on plane s-h at step s, on
+// the tile grown by H-1-h voxels, so that after the last half-sweep the
+// owned tile holds exactly the values of H successive reference iterations.
+// Red-black colouring makes each half-sweep a Jacobi step within one colour,
+// so reading src and writing dst (no in-place hazard between workgroups
+// that recompute each other's halos) gives the in-place result.
+//
+// Workgroups are numbered so that consecutive tiles land on the same XCD
+// (hardware dispatch sends workgroup i to XCD i mod 8): neighbouring tiles
+// then share their halo lines in that XCD's L2.
+constexpr int kSweepThreads = 256;
+
+struct SweepArgs {
+    const float* src;
+    float* dst;
+    const uint8_t* codes;
+    const float* hm;
+    int ncls;
+    Vol v;
+    float w;
+    int colour0;
+    int TY, TZ, TX;
+    int nTz, nTy, nTx, total;
+};
+
+template <int H>
+__host__ __device__ constexpr int ring_planes() { return H + 2; }
+
+template <int H, int KIND>
+__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NP = ring_planes<H>();
+    const int EY = a.TY + 2 * H, EZ = a.TZ + 2 * H, EP = EY * EZ;
+    float* sF = reinterpret_cast<float*>(smem);
+    uint8_t* sC = reinterpret_cast<uint8_t*>(sF + NP * EP);
+    float* sT = reinterpret_cast<float*>(sC + ((NP * EP + 15) & ~15));
+
+    // XCD-aware tile numbering (grid is a multiple of 8)
+    const int per = gridDim.x >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= a.total) return;
+    int t = logical;
+    const int tz = t % a.nTz; t /= a.nTz;
+    const int ty = t % a.nTy; t /= a.nTy;
+    const int tx = t % a.nTx;
+    const int b = t / a.nTx;
+
+    const Vol v = a.v;
+    const float* __restrict__ S = a.src + (size_t)b * v.sB;
+    float* __restrict__ D = a.dst + (size_t)b * v.sB;
+    const uint8_t* __restrict__ Cg = a.codes + (size_t)b * v.sB;
+
+    const int y0 = 1 + ty * a.TY, z0 = 1 + tz * a.TZ, x0 = 1 + tx * a.TX;
+    const int y1 = min(y0 + a.TY, v.Y + 1), z1 = min(z0 + a.TZ, v.Z + 1), x1 = min(x0 + a.TX, v.X + 1);
+    const int ey0 = y0 - H, ez0 = z0 - H;  // padded coordinate of LDS (0,0)
+    const int ly0 = max(ey0, 0), ly1 = min(y1 + H, v.Y + 2);
+    const int lz0 = max(ez0, 0), lz1 = min(z1 + H, v.Z + 2);
+    const int lx1 = min(x1 + H, v.X + 2);
+    const int nly = ly1 - ly0, nlz = lz1 - lz0;
+    const int tid = threadIdx.x;
+
+    // lookup tables
+    if (KIND == TF_KIND_MULTIPHASE) {
+        for (int i = tid; i < a.ncls * a.ncls; i += kSweepThreads) sT[i] = a.hm[i];
+    }
+
+    const int nload = nly * nlz;
+    const float rinv_load = 1.0f / (float)nlz;
+    auto load_plane = [&](int p) {
+        float* dF = sF + (p % NP) * EP;
+        uint8_t* dC = sC + (p % NP) * EP;
+        const int gbase = p * v.sX;
+        for (int i = tid; i < nload; i += kSweepThreads) {
+            int r = (int)((float)i * rinv_load);
+            if (r * nlz > i) --r;
+            else if ((r + 1) * nlz <= i) ++r;
+            const int zz = lz0 + (i - r * nlz);
+            const int yy = ly0 + r;
+            const int g = gbase + yy * v.sY + zz;
+            const int l = (yy - ey0) * EZ + (zz - ez0);
+            dF[l] = S[g];
+            dC[l] = Cg[g];
+        }
+    };
+
+    const int S0 = max(x0 - (H - 1), 1);
+    const int S1 = (x1 - 1) + (H - 1);
+    load_plane(S0 - 1);
+    if (S0 < lx1) load_plane(S0);
+
+    for (int s = S0; s <= S1; ++s) {
+        if (s + 1 < lx1) load_plane(s + 1);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            const int p = s - h;
+            const int grow = H - 1 - h;
+            if (p >= max(x0 - grow, 1) && p < min(x1 + grow, v.X + 1)) {
+                const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
+                const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
+                const int nry = ry1 - ry0, nrz = rz1 - rz0;
+                const int npair = (nrz + 1) >> 1;
+                const int nitem = nry * npair;
+                const float rinv = 1.0f / (float)npair;
+                const int colour = a.colour0 ^ (h & 1);
+                float* P = sF + (p % NP) * EP;
+                const float* Pm = sF + ((p - 1) % NP) * EP;
+                const float* Pp = sF + ((p + 1) % NP) * EP;
+                const uint8_t* CP = sC + (p % NP) * EP;
+                const uint8_t* CPm = sC + ((p - 1) % NP) * EP;
+                const uint8_t* CPp = sC + ((p + 1) % NP) * EP;
+                for (int i = tid; i < nitem; i += kSweepThreads) {
+                    int r = (int)((float)i * rinv);
+                    if (r * npair > i) --r;
+                    else if ((r + 1) * npair <= i) ++r;
+                    const int k = i - r * npair;
+                    const int yy = ry0 + r;
+                    // interior colour (x-1)+(y-1)+(z-1) == colour  <=>  (p+yy+zz+1)&1 == colour
+                    int zz = rz0 + 2 * k;
+                    zz += ((p + yy + zz + 1) & 1) ^ colour;
+                    if (zz >= rz1) continue;
+                    const int l = (yy - ey0) * EZ + (zz - ez0);
+                    Nbr n;
+                    n.c = P[l];
+                    n.xp = Pp[l];
+                    n.xm = Pm[l];
+                    n.yp = P[l + EZ];
+                    n.ym = P[l - EZ];
+                    n.zp = P[l + 1];
+                    n.zm = P[l - 1];
+                    float out;
+                    if (KIND == TF_KIND_SOLVER) {
+                        out = update_solver(n, CP[l], a.w);
+                    } else {
+                        const float* row = sT + CP[l] * a.ncls;
+                        out = update_multiphase(n, row[CPp[l]], row[CPm[l]], row[CP[l + EZ]],
+                                                row[CP[l - EZ]], row[CP[l + 1]], row[CP[l - 1]],
+                                                p == 1, p == v.X, a.w);
+                    }
+                    P[l] = out;
+                }
+            }
+            __syncthreads();
+        }
+        // plane s-H+1 has had all H half-sweeps; write its owned tile
+        const int q = s - (H - 1);
+        if (q >= x0 && q < x1) {
+            const float* P = sF + (q % NP) * EP;
+            const int oy = y1 - y0, oz = z1 - z0;
+            const int nst = oy * oz;
+            const float rinv = 1.0f / (float)oz;
+            for (int i = tid; i < nst; i += kSweepThreads) {
+                int r = (int)((float)i * rinv);
+                if (r * oz > i) --r;
+                else if ((r + 1) * oz <= i) ++r;
+                const int zz = z0 + (i - r * oz);
+                const int yy = y0 + r;
+                D[q * v.sX + yy * v.sY + zz] = P[(yy - ey0) * EZ + (zz - ez0)];
+            }
+        }
+    }
+}
+
+static void default_tile(int H, tf_tile* t) {
+    t->tz = 64;
+    t->ty = H <= 2 ? 16 : (H <= 4 ? 16 : 8);
+    t->tx = 64;
+}
+
+static size_t sweep_lds_bytes(int H, int kind, int ncls, const tf_tile& t) {
+    size_t EP = (size_t)(t.ty + 2 * H) * (t.tz + 2 * H);
+    size_t NP = (size_t)H + 2;
+    size_t bytes = NP * EP * sizeof(float) + ((NP * EP + 15) & ~(size_t)15);
+    if (kind == TF_KIND_MULTIPHASE) bytes += (size_t)ncls * ncls * sizeof(float);
+    return bytes;
+}
+
+template <int H, int KIND>
+static int launch_sweep(const SweepArgs& a, size_t lds, int grid, hipStream_t st) {
+    static bool attr_set = false;
+    if (!attr_set && lds > 64 * 1024) {
+        // opt in to more than 64 KiB of dynamic LDS; if the runtime refuses,
+        // the launch below reports the error
+        if (hipFuncSetAttribute((const void*)k_sweep<H, KIND>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+            (void)hipGetLastError();
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_sweep<H, KIND>), dim3(grid), dim3(kSweepThreads), lds, st, a);
+    TF_LAUNCH_CHECK("k_sweep");
+    return 0;
+}
+
+template <int KIND>
+static int dispatch_sweep(int H, const SweepArgs& a, size_t lds, int grid, hipStream_t st) {
+    switch (H) {
+        case 1: return launch_sweep<1, KIND>(a, lds, grid, st);
+        case 2: return launch_sweep<2, KIND>(a, lds, grid, st);
+        case 3: return launch_sweep<3, KIND>(a, lds, grid, st);
+        case 4: return launch_sweep<4, KIND>(a, lds, grid, st);
+        case 5: return launch_sweep<5, KIND>(a, lds, grid, st);
+        case 6: return launch_sweep<6, KIND>(a, lds, grid, st);
+        case 7: return launch_sweep<7, KIND>(a, lds, grid, st);
+        case 8: return launch_sweep<8, KIND>(a, lds, grid, st);
+        default: return fail(-1, "n_half=%d out of range 1..%d", H, TF_MAX_FUSED_HALF_SWEEPS);
+    }
+}
+
+extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, const float* hm_lut,
+                        int n_classes, int kind, int B, int X, int Y, int Z, float omega,
+                        int first_colour, int n_half, const tf_tile* tile, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!src || !dst || !codes) return fail(-1, "null pointer argument");
+    if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
+    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE) return fail(-1, "unknown kind %d", kind);
+    if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 64))
+        return fail(-1, "multiphase sweep needs hm_lut and 1..64 classes (got %d)", n_classes);
+    if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
+    if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
+        return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
+    tf_tile t;
+    default_tile(n_half, &t);
+    if (tile) {
+        if (tile->ty > 0) t.ty = tile->ty;
+        if (tile->tz > 0) t.tz = tile->tz;
+        if (tile->tx > 0) t.tx = tile->tx;
+    }
+    if (t.ty < 1 || t.tz < 1 || t.tx < 1 || t.ty > 256 || t.tz > 1024 || t.tx > 4096)
+        return fail(-1, "invalid tile %dx%dx%d", t.ty, t.tz, t.tx);
+    size_t lds = sweep_lds_bytes(n_half, kind, n_classes, t);
+    if (lds > 160 * 1024)
+        return fail(-1, "tile %dx%dx%d with n_half=%d needs %zu B of LDS (> 160 KiB)", t.ty, t.tz,
+                    t.tx, n_half, lds);
+    SweepArgs a;
+    a.src = src; a.dst = dst; a.codes = codes; a.hm = hm_lut; a.ncls = n_classes;
+    a.v = make_vol(X, Y, Z);
+    a.w = omega;
+    a.colour0 = first_colour;
+    a.TY = t.ty; a.TZ = t.tz; a.TX = t.tx;
+    a.nTz = (Z + t.tz - 1) / t.tz;
+    a.nTy = (Y + t.ty - 1) / t.ty;
+    a.nTx = (X + t.tx - 1) / t.tx;
+    long long total = (long long)a.nTz * a.nTy * a.nTx * B;
+    if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
+    a.total = (int)total;
+    int grid = (int)(((total + 7) / 8) * 8);
+    hipStream_t st = (hipStream_t)stream;
+    if (kind == TF_KIND_SOLVER) return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, lds, grid, st);
+    return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, lds, grid, st);
+}
+
+// ---------------------------------------------------------------------------
+// per-layer float64 sums for the convergence check
+// ---------------------------------------------------------------------------
+// One block per (x, b).  Each thread accumulates a fixed strided subset of
+// the layer in float64; the block then reduces in a fixed tree, so the result
+// does not depend on scheduling.
+constexpr int kSumThreads = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restrict__ F,
+                                                            const uint8_t* __restrict__ C,
+                                                            const float* __restrict__ hm, int ncls,
+                                                            Vol v, double* __restrict__ flux,
+                                                            double* __restrict__ conc) {
+    const int x = blockIdx.x, b = blockIdx.y;
+    const float* V = F + (size_t)b * v.sB;
+    const uint8_t* Cv = C + (size_t)b * v.sB;
+    const bool has_face = x < v.X - 1;
+    double accC = 0.0, accF = 0.0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int y = wave; y < v.Y; y += kSumThreads / 64) {
+        const int row = (x + 1) * v.sX + (y + 1) * v.sY + 1;
+        for (int z = lane; z < v.Z; z += 64) {
+            const int o = row + z;
+            const float c = V[o];
+            accC += (double)c;
+            if (has_face) {
+                const float cn = V[o + v.sX];
+                float d = cn - c;
+                if (KIND == TF_KIND_SOLVER) {
+                    if (Cv[o] == 0 || Cv[o + v.sX] == 0) d = 0.0f;
+                } else {
+                    d = hm[Cv[o] * ncls + Cv[o + v.sX]] * d;
+                }
+                accF += (double)d;
+            }
+        }
+    }
+    __shared__ double redC[kSumThreads / 64], redF[kSumThreads / 64];
+    accC = wave_sum(accC);
+    accF = wave_sum(accF);
+    if (lane == 0) {
+        redC[wave] = accC;
+        redF[wave] = accF;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sc = 0.0, sf = 0.0;
+        for (int i = 0; i < kSumThreads / 64; ++i) {
+            sc += redC[i];
+            sf += redF[i];
+        }
+        conc[(size_t)b * v.X + x] = sc;
+        if (has_face) flux[(size_t)b * (v.X - 1) + x] = sf;
+    }
+}
+
+extern "C" int tf_layer_sums(const float* field, const uint8_t* codes, const float* hm_lut,
+                             int n_classes, int kind, int B, int X, int Y, int Z, double* flux_sum,
+                             double* conc_sum, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!field || !codes || !conc_sum || (X > 1 && !flux_sum)) return fail(-1, "null pointer argument");
+    if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 256))
+        return fail(-1, "multiphase sums need hm_lut and 1..256 classes");
+    if (X > 65535 || B > 65535) return fail(-1, "grid too large for layer sums");
+    Vol v = make_vol(X, Y, Z);
+    hipStream_t st = (hipStream_t)stream;
+    dim3 grid(X, B);
+    if (kind == TF_KIND_SOLVER)
+        hipLaunchKernelGGL(k_layer_sums<TF_KIND_SOLVER>, grid, dim3(kSumThreads), 0, st, field,
+                           codes, hm_lut, n_classes, v, flux_sum, conc_sum);
+    else if (kind == TF_KIND_MULTIPHASE)
+        hipLaunchKernelGGL(k_layer_sums<TF_KIND_MULTIPHASE>, grid, dim3(kSumThreads), 0, st, field,
+                           codes, hm_lut, n_classes, v, flux_sum, conc_sum);
+    else
+        return fail(-1, "unknown kind %d", kind);
+    TF_LAUNCH_CHECK("k_layer_sums");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// host: does a 6-connected conductive cluster span x?
+// ---------------------------------------------------------------------------
+// Breadth-first search from every conductive voxel of the x=0 face; the
+// volume spans iff the search reaches the x=X-1 face.  Equivalent to asking
+// whether some 6-connected component label occurs on both x faces.
+extern "C" int tf_spans_x(const uint8_t* mask, int X, int Y, int Z) {
+    if (!mask) return fail(-1, "null mask");
+    if (X < 1 || Y < 1 || Z < 1) return fail(-1, "invalid shape %dx%dx%d", X, Y, Z);
+    const long long n = (long long)X * Y * Z;
+    if (n >= (1LL << 31)) return fail(-1, "volume too large for spans test");
+    const int sy = Z, sx = Y * Z;
+    std::vector<uint8_t> seen((size_t)n, 0);
+    std::vector<int> queue;
+    queue.reserve(1024);
+    for (int i = 0; i < sx; ++i)
+        if (mask[i]) {
+            seen[i] = 1;
+            queue.push_back(i);
+        }
+    size_t head = 0;
+    while (head < queue.size()) {
+        const int o = queue[head++];
+        const int x = o / sx, r = o - x * sx, y = r / sy, z = r - y * sy;
+        if (x == X - 1) return 1;
+        int nb[6];
+        int k = 0;
+        if (x + 1 < X) nb[k++] = o + sx;
+        if (x >= 1) nb[k++] = o - sx;
+        if (y + 1 < Y) nb[k++] = o + sy;
+        if (y >= 1) nb[k++] = o - sy;
+        if (z + 1 < Z) nb[k++] = o + 1;
+        if (z >= 1) nb[k++] = o - 1;
+        for (int j = 0; j < k; ++j) {
+            const int q = nb[j];
+            if (mask[q] && !seen[q]) {
+                seen[q] = 1;
+                queue.push_back(q);
+            }
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,573 @@
+"""Host side of the MI355X steady-state diffusion solve.
+
+The classes mirror the reference's solver API (taufactor/taufactor.py):
+constructor signatures, attributes, ``solve()`` arguments, return values,
+printed messages and input errors are those of ``SORSolver`` (:15-269),
+``ThroughTransportSolver`` (:272-352), ``Solver`` (:355-419) and
+``MultiPhaseSolver`` (:508-620).  The iteration itself runs in libtauhip
+(hand-written gfx950 kernels, include/tauhip.h); Python only validates
+input, launches, and evaluates the per-layer sums every 100 iterations.
+
+Differences from the reference, all deliberate:
+  * there is no CPU code path: ``device`` must name a HIP GPU (``'cuda'`` on
+    ROCm); a CPU device, or no GPU, raises ``RuntimeError`` instead of
+    falling back to CPU with a warning (taufactor/taufactor.py:207-216);
+  * the convergence metrics are evaluated from float64 per-layer sums
+    (the reference averages in float32, whose flux-uniformity test cancels
+    catastrophically); public result arrays are float32 as in the reference;
+  * a NaN mean flux is still treated as converged (taufactor/taufactor.py:329)
+    but raises a ``RuntimeWarning`` instead of passing silently;
+  * ``walltime`` includes the completion of the last kernel.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from abc import ABC, abstractmethod
+from timeit import default_timer as timer
+
+import numpy as np
+import torch
+
+from . import _lib
+
+__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "MultiPhaseSolver"]
+
+# half-sweeps fused per HBM pass unless a solver is told otherwise
+DEFAULT_HALF_SWEEPS_PER_PASS = 4
+# reference check cadence (taufactor/taufactor.py:184) and tau tolerance (:148)
+CHECK_EVERY = 100
+TAU_TOL = 2e-3
+
+
+class SORSolver(ABC):
+    """Template of the red-black SOR solve (taufactor/taufactor.py:15-269).
+
+    Args:
+        img: labelled input image, [X,Y], [X,Y,Z] or [B,X,Y,Z] numpy array.
+        omega: over-relaxation factor; default 2 - pi/(1.5*Nx).
+        precision: must be torch.float32 (the kernels compute in float32).
+        device: a HIP device ('cuda', 'cuda:N').
+    """
+
+    kind: int = _lib.KIND_SOLVER
+
+    def __init__(self, img: np.ndarray, omega: float | None = None, precision=None, device="cuda"):
+        self.cpu_img = self._expand_to_4d(img)
+        self.batch_size, self.Nx, self.Ny, self.Nz = self.cpu_img.shape
+        self.device = self._init_device(device)
+        self.precision = precision or torch.float
+        if self.precision != torch.float32:
+            raise ValueError("taufactor_amd computes in float32 only (precision=torch.float)")
+
+        if omega is None:
+            omega = 2 - torch.pi / (1.5 * self.Nx)
+        self.omega = omega
+        # the reference rounds omega to float32 when building its chequerboard
+        # (taufactor/taufactor.py:224-225)
+        self._w32 = float(np.float32(omega))
+
+        self._lib = _lib.load()
+        self.half_sweeps_per_pass = DEFAULT_HALF_SWEEPS_PER_PASS
+        self.tile = None  # (ty, tz, tx) or None for the library default
+
+        self._fields = None  # two padded float32 buffers (ping-pong)
+        self._cur = 0
+        self._codes = None
+        self._hm_lut = None
+        self._n_classes = 0
+        self._spans_cache = {}
+        with torch.cuda.device(self.device):
+            self._setup_device()
+            B, X = self.batch_size, self.Nx
+            self._flux_sum = torch.zeros((B, max(X - 1, 1)), dtype=torch.float64, device=self.device)
+            self._conc_sum = torch.zeros((B, X), dtype=torch.float64, device=self.device)
+
+        self.converged = False
+        self.old_tau = 0
+        self.iter = 0
+        self.tau = None
+        self.tau_x = None
+        self.D_eff = None
+
+    # ---------------- hooks the subclasses implement ----------------
+    @abstractmethod
+    def _setup_device(self) -> None:
+        """Pack codes, write both field buffers, set self.vol_x."""
+
+    @abstractmethod
+    def compute_metrics(self):
+        """Defines tau and relative error"""
+
+    def return_mask(self, img):
+        raise NotImplementedError
+
+    def apply_boundary_conditions(self):
+        """Dirichlet in x and no-flux in y and z: held by the ghost layers."""
+
+    # ---------------- device state ----------------
+    @property
+    def field(self) -> torch.Tensor:
+        """Current padded field [B, Nx+2, Ny+2, Nz+2] (float32, on device)."""
+        return self._fields[self._cur]
+
+    @field.setter
+    def field(self, value: torch.Tensor) -> None:
+        value = torch.as_tensor(value, dtype=torch.float32, device=self.device)
+        for f in self._fields:
+            f.copy_(value)
+
+    @property
+    def cb(self):
+        """[omega*cb, omega*(1-cb)] float32 chequerboards (taufactor/taufactor.py:218-225)."""
+        x = torch.arange(self.Nx, device=self.device).view(-1, 1, 1)
+        y = torch.arange(self.Ny, device=self.device).view(1, -1, 1)
+        z = torch.arange(self.Nz, device=self.device).view(1, 1, -1)
+        even = ((x + y + z) % 2 == 0).to(torch.float64)
+        return [(self.omega * even).to(torch.float32), (self.omega * (1 - even)).to(torch.float32)]
+
+    def _stream(self) -> int:
+        return _lib.stream_handle(self.device)
+
+    def _xinit(self) -> torch.Tensor:
+        # linear initial profile (ThroughTransportSolver.init_field,
+        # taufactor/taufactor.py:284-286), computed by torch.linspace in float32
+        sh = 1 / (2 * self.Nx)
+        vec = torch.linspace(self.top_bc + sh, self.bot_bc - sh, self.Nx, dtype=torch.float32)
+        return vec.to(self.device)
+
+    def _advance(self, n_half: int) -> None:
+        """Run n_half reference iterations (half-sweeps) on the device."""
+        lib = self._lib
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        tile = None
+        if self.tile is not None:
+            tile = _lib.Tile(*self.tile)
+        stream = self._stream()
+        codes = _lib.ptr(self._codes)
+        hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
+        it = self.iter
+        per = max(1, min(int(self.half_sweeps_per_pass), _lib.MAX_FUSED_HALF_SWEEPS))
+        while n_half > 0:
+            h = min(per, n_half)
+            src = self._fields[self._cur]
+            dst = self._fields[1 - self._cur]
+            st = lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes, self.kind,
+                              B, X, Y, Z, self._w32, it & 1, h,
+                              None if tile is None else tile, stream)
+            _lib.check(st, "tf_sweep")
+            self._cur = 1 - self._cur
+            it += h
+            n_half -= h
+
+    def _layer_sums(self):
+        """Per-layer float64 flux and concentration sums, copied to the host."""
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        st = self._lib.tf_layer_sums(
+            _lib.ptr(self.field), _lib.ptr(self._codes),
+            _lib.ptr(self._hm_lut) if self._hm_lut is not None else None,
+            self._n_classes, self.kind, B, X, Y, Z,
+            _lib.ptr(self._flux_sum), _lib.ptr(self._conc_sum), self._stream())
+        _lib.check(st, "tf_layer_sums")
+        flux = self._flux_sum.cpu().numpy()[:, : X - 1]
+        conc = self._conc_sum.cpu().numpy()
+        return flux, conc
+
+    # ---------------- convergence ----------------
+    def check_convergence(self, verbose, conv_crit, plot_interval):
+        """taufactor/taufactor.py:109-153; decisions are taken in float64."""
+        tau, relative_error = self.compute_metrics()
+
+        if verbose == "per_iter":
+            i = np.argmax(relative_error)
+            print(f"Iter: {self.iter}, conv error: {abs(relative_error[i]):.3E}, "
+                  f"tau: {tau[i]:.5f} (batch element {i})")
+        if verbose == "debug":
+            self.tau_t.append(tau.astype(np.float32))
+
+        if not np.all(relative_error < conv_crit):
+            self.old_tau = tau
+            self.tau = tau.astype(np.float32)
+            return False
+
+        tau_error = np.max(np.abs(tau - self.old_tau))
+        if not tau_error < TAU_TOL:
+            self.old_tau = tau
+            self.tau = tau.astype(np.float32)
+            return False
+
+        tau = tau.copy()
+        tau[tau == 0] = np.inf
+        self.tau = tau.astype(np.float32)
+        return True
+
+    # ---------------- main loop -------------------
+    def solve(self, iter_limit=10000, verbose=True, conv_crit=1e-2, plot_interval=10):
+        """
+        Solve steady-state with SOR solver (taufactor/taufactor.py:156-191)
+
+        :param iter_limit: max iterations before aborting
+        :param verbose: Set to 'True', 'per_iter' or 'debug' for more feedback
+        :param conv_crit: convergence criteria, minimum percent difference between
+        max and min flux through a given layer
+        :return: tau_x, or tau if no convergence check ran
+        """
+        if verbose:
+            torch.cuda.reset_peak_memory_stats(device=self.device)
+        if verbose == "debug":
+            self.tau_t = []
+
+        with torch.no_grad(), torch.cuda.device(self.device):
+            start = timer()
+            while not self.converged and self.iter < iter_limit:
+                self.apply_boundary_conditions()
+                next_check = (self.iter // CHECK_EVERY + 1) * CHECK_EVERY
+                n = min(next_check, iter_limit) - self.iter
+                self._advance(n)
+                self.iter += n
+                if self.iter % CHECK_EVERY == 0:
+                    self.converged = self.check_convergence(verbose, conv_crit, plot_interval)
+            torch.cuda.synchronize(self.device)
+            self.walltime = timer() - start
+            self._end_simulation(self.iter, verbose)
+            if self.tau_x is None:
+                return self.tau
+            return self.tau_x
+
+    # ---------------- helpers ----------------
+    @staticmethod
+    def _expand_to_4d(img: np.ndarray) -> np.ndarray:
+        if not isinstance(img, np.ndarray):
+            raise TypeError("Error: input image must be a NumPy array!")
+        if img.ndim == 2:
+            img = img[..., None]
+        if img.ndim == 3:
+            img = img[None, ...]
+        if img.ndim != 4:
+            raise ValueError("expected [B, X, Y, Z]")
+        return img
+
+    @staticmethod
+    def _init_device(device) -> torch.device:
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise RuntimeError(
+                f"taufactor_amd runs on a HIP GPU only (device='cuda' on ROCm); got device='{device}'")
+        if not torch.cuda.is_available():
+            raise RuntimeError("taufactor_amd needs a HIP GPU, but torch.cuda.is_available() is False")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        return dev
+
+    def _end_simulation(self, iterations: int, verbose: bool):
+        if self.converged:
+            msg = "converged to"
+        else:
+            print("Warning: not converged")
+            msg = "unconverged value of tau"
+
+        if verbose:
+            per_iter = self.walltime / iterations if iterations else float("nan")
+            print(f"{msg}: {self.tau} after: {iterations} iterations in: "
+                  f"{np.around(self.walltime, 4)} s "
+                  f"({np.around(per_iter, 4)} s/iter)")
+            print(f"GPU-RAM currently {torch.cuda.memory_allocated(device=self.device) / 1e6:.2f} MB "
+                  f"(max allocated {torch.cuda.max_memory_allocated(device=self.device) / 1e6:.2f} MB; "
+                  f"{torch.cuda.max_memory_reserved(device=self.device) / 1e6:.2f} MB reserved)")
+
+
+class ThroughTransportSolver(SORSolver):
+    """Dirichlet x boundaries at (-0.5, +0.5), zero flux on y and z
+    (taufactor/taufactor.py:272-352)."""
+
+    def __init__(self, img, omega=None, precision=None, device="cuda"):
+        self.top_bc, self.bot_bc = (-0.5, 0.5)
+        super().__init__(img, omega, precision, device)
+
+    def _spans(self, b: int) -> bool:
+        """Does a 6-connected conductive cluster of batch element b span x?
+        (extract_through_feature(...)[1] != 0, taufactor/taufactor.py:320-322)"""
+        if b not in self._spans_cache:
+            mask = np.ascontiguousarray(np.isin(self.cpu_img[b], self.conductive_labels), dtype=np.uint8)
+            X, Y, Z = mask.shape
+            r = self._lib.tf_spans_x(mask.ctypes.data, X, Y, Z)
+            if r < 0:
+                _lib.check(r, "tf_spans_x")
+            self._spans_cache[b] = bool(r)
+        return self._spans_cache[b]
+
+    def compute_metrics(self):
+        """taufactor/taufactor.py:293-331, from float64 per-layer sums."""
+        flux_sum, conc_sum = self._layer_sums()
+        n2 = self.Ny * self.Nz
+        flux_1d = flux_sum / n2
+        fl_max = np.max(flux_1d, axis=1)
+        fl_min = np.min(flux_1d, axis=1)
+        mean_fl = np.mean(flux_1d, axis=1)
+        relative_error = np.divide(fl_max - fl_min, fl_max,
+                                   out=np.full_like(fl_max, np.nan), where=fl_max != 0)
+
+        D_rel = mean_fl * self.Nx / abs(self.top_bc - self.bot_bc)
+        D_mean = np.asarray(self.D_mean, dtype=np.float64)
+        tau = np.divide(D_mean, D_rel, out=np.full_like(D_rel, np.nan), where=D_rel != 0)
+
+        vol_x = self.vol_x.astype(np.float64)
+        c_x = conc_sum / n2
+        c_x = np.divide(c_x, vol_x, out=np.zeros_like(vol_x), where=vol_x != 0)
+        flux_from_c = c_x[:, 1:] - c_x[:, :-1]
+        flux_from_c[vol_x[:, 1:] == 0] = 0
+        flux_from_c[vol_x[:, :-1] == 0] = 0
+        eps = 0.5 * (vol_x[:, :-1] + vol_x[:, 1:])
+        tau_x = np.divide(eps * flux_from_c, flux_1d,
+                          out=np.full_like(flux_from_c, np.nan), where=flux_1d != 0)
+
+        for b in range(self.batch_size):
+            if (fl_min[b] == 0) or (fl_max[b] == 0) or (mean_fl[b] == 0):
+                if not self._spans(b):
+                    print(f"Warning: batch element {b} has no percolating path!")
+                    relative_error[b] = 0
+                    D_rel[b] = 0
+                    tau[b] = 0
+                    tau_x[b, :] = 0
+        nan_mean = np.isnan(mean_fl)
+        if np.any(nan_mean):
+            warnings.warn(f"NaN mean flux in batch element(s) {np.nonzero(nan_mean)[0].tolist()}; "
+                          "treated as converged as in the reference", RuntimeWarning)
+        relative_error[nan_mean] = 0
+
+        self.flux_1d = flux_1d.astype(np.float32)
+        self.c_x = c_x.astype(np.float32)
+        self.tau_x = tau_x.astype(np.float32)
+        self.D_rel = D_rel
+        self.D_eff = (self.D_0 * D_rel).astype(np.float32)
+        return tau, relative_error
+
+
+def _label_table(img: np.ndarray):
+    """(labels, counts[B, L]) of the label values of a [B,X,Y,Z] image."""
+    flat = img.reshape(img.shape[0], -1)
+    if img.dtype.kind in "iub" or (img.dtype.kind == "f" and img.size and
+                                   np.array_equal(img, np.floor(img))):
+        lo = img.min() if img.size else 0
+        hi = img.max() if img.size else 0
+        if lo >= 0 and hi < 65536:
+            idx = flat.astype(np.int64, copy=False)
+            counts = np.stack([np.bincount(r, minlength=int(hi) + 1) for r in idx])
+            present = np.nonzero(counts.sum(0))[0]
+            return present.astype(img.dtype), counts[:, present]
+    labels, inv = np.unique(flat, return_inverse=True)
+    inv = inv.reshape(flat.shape)
+    counts = np.stack([np.bincount(r, minlength=len(labels)) for r in inv])
+    return labels, counts
+
+
+class Solver(ThroughTransportSolver):
+    """Two-phase (binary) through-transport solver (taufactor/taufactor.py:355-419).
+
+    Args:
+        img (numpy.ndarray): Binary image with labels in ``{0, 1}``.
+        omega (float, optional): over-relaxation factor.
+        D_0 (float, optional): Reference (mean) diffusivity. Defaults to ``1``.
+        device (str | torch.device, optional): HIP device. Defaults to ``'cuda'``.
+
+    Raises:
+        ValueError: If labels are not strictly in ``{0, 1}``.
+    """
+
+    kind = _lib.KIND_SOLVER
+
+    def __init__(self, img, omega=None, D_0=1, device="cuda"):
+        self._check_binary_labels(img)
+        self.conductive_labels = [1]
+        super().__init__(img, omega=omega, device=device)
+        self.D_0 = D_0
+        self.D_mean = np.mean(self.vol_x, axis=1)
+
+    def _check_binary_labels(self, img):
+        arr = np.asarray(img)
+        if arr.size and bool(np.all((arr == 0) | (arr == 1))):
+            return
+        if len(np.unique(img)) > 2 or \
+           np.unique(img).max() not in [0, 1] or \
+           np.unique(img).min() not in [0, 1]:
+            raise ValueError(
+                "Input image must only contain 0s and 1s. "
+                "Your image must be segmented to use this tool. "
+                "If your image has been segmented, ensure your labels are "
+                "0 for non-conductive and 1 for conductive phase. "
+                f"Your image has the following labels: {np.unique(img)}. "
+                "If you have more than one conductive phase, use the multi-phase solver.")
+
+    def _setup_device(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        mask = torch.from_numpy(np.ascontiguousarray(self.cpu_img != 0, dtype=np.uint8)).to(self.device)
+        shape = (B, X + 2, Y + 2, Z + 2)
+        self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
+        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        count = torch.empty((B, X), dtype=torch.int32, device=self.device)
+        xinit = self._xinit()
+        st = self._lib.tf_setup_solver(_lib.ptr(mask), _lib.ptr(xinit), _lib.ptr(self._codes),
+                                       _lib.ptr(self._fields[0]), _lib.ptr(self._fields[1]),
+                                       _lib.ptr(count), B, X, Y, Z, self._stream())
+        _lib.check(st, "tf_setup_solver")
+        self.vol_x = count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+
+    @property
+    def factor(self) -> torch.Tensor:
+        """Neighbour count, inf where the update is void (taufactor/taufactor.py:402-410)."""
+        c = self._codes[:, 1:-1, 1:-1, 1:-1].to(torch.float32)
+        return torch.where(c == 0, torch.full_like(c, math.inf), c)
+
+    def vertical_flux(self) -> torch.Tensor:
+        """taufactor/taufactor.py:412-419"""
+        f = self.field
+        vert_flux = f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1]
+        fac = self.factor
+        vert_flux[fac[:, 0:-1] > 8] = 0
+        vert_flux[fac[:, 1:] > 8] = 0
+        return vert_flux
+
+
+class MultiPhaseSolver(ThroughTransportSolver):
+    """Multi-phase SOR solver with per-phase diffusivity and harmonic-mean
+    face conductances (taufactor/taufactor.py:508-620).
+
+    Args:
+        img (numpy.ndarray): Labeled image.
+        diffusivities (dict[int, float], optional): label -> diffusivity.
+            Labels not provided are assumed isolating. Defaults to ``{0: 0, 1: 1}``.
+        D_scaling (float, optional): reference diffusivity ``D_0``.
+        omega (float, optional): over-relaxation factor.
+        device (str | torch.device, optional): HIP device. Defaults to ``'cuda'``.
+
+    Raises:
+        TypeError: non-dict diffusivities or non-integer labels.
+        ValueError: If any diffusivity is negative or non-finite.
+    """
+
+    kind = _lib.KIND_MULTIPHASE
+    MAX_CLASSES = 64
+
+    def __init__(self, img, diffusivities=None, D_scaling=1, omega=None, device="cuda"):
+        if diffusivities is None:
+            diffusivities = {0: 0, 1: 1}
+        if not isinstance(diffusivities, dict):
+            raise TypeError("diffusivities must be a dictionary mapping phase labels to diffusivities")
+        for phase, D_p in diffusivities.items():
+            if not isinstance(phase, (int, np.integer)):
+                raise TypeError(f"Phase label must be integer, got {type(phase).__name__}")
+            D_p = float(D_p)
+            if (not np.isfinite(D_p)) or (D_p < 0):
+                raise ValueError(f"Diffusivity for label {phase} must be finite and >= 0, got {D_p}")
+        self.Ds = diffusivities
+
+        missing_labels = sorted(int(lbl) for lbl in np.unique(img) if lbl not in self.Ds)
+        if missing_labels:
+            warnings.warn(
+                "No diffusivity provided for phase label(s) "
+                f"{missing_labels}; assuming these phases are isolating.",
+                UserWarning,
+            )
+            for lbl in missing_labels:
+                self.Ds[lbl] = 0.0
+        self.conductive_labels = [lbl for lbl, D_p in self.Ds.items() if D_p > 0]
+
+        super().__init__(img, omega=omega, device=device)
+        self.VF = {int(p): self._label_counts[:, i] / float(self.Nx * self.Ny * self.Nz)
+                   for i, p in enumerate(self._labels)}
+        self.D_0 = D_scaling
+        self.D_mean = np.sum([self.VF[z] * self.Ds.get(z, 0.0) for z in self.VF], axis=0)
+
+    def _setup_device(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        labels, counts = _label_table(self.cpu_img)
+        self._labels, self._label_counts = labels, counts
+        # diffusivity per label as the reference's float32 diff_map holds it
+        d_label = np.array([np.float32(self.Ds.get(lbl.item(), 0.0)) for lbl in labels], dtype=np.float32)
+        # classes: 0 = zero diffusivity (and ghosts), then one per distinct value
+        pos = sorted(set(float(d) for d in d_label if d > 0))
+        if len(pos) + 1 > self.MAX_CLASSES:
+            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct non-zero diffusivities are supported")
+        d_cls = np.array([0.0] + pos, dtype=np.float32)
+        cls_of_label = np.array([0 if d == 0 else 1 + pos.index(float(d)) for d in d_label], dtype=np.uint8)
+        # face conductance table: hm(a,b) = 2*a*b/(a+b), 0 where a+b == 0
+        # (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
+        a = d_cls[:, None]
+        b = d_cls[None, :]
+        denom = a + b
+        with np.errstate(divide="ignore", invalid="ignore"):
+            hm = np.where(denom > 0, (2 * a * b) / denom, np.float32(0)).astype(np.float32)
+        self._d_cls = d_cls
+        self._n_classes = len(d_cls)
+        cls_img = self._class_image(labels, cls_of_label)
+        cls_t = torch.from_numpy(cls_img).to(self.device)
+        self._hm_lut = torch.from_numpy(np.ascontiguousarray(hm)).to(self.device)
+        cond = torch.from_numpy((d_cls > 0).astype(np.uint8)).to(self.device)
+        shape = (B, X + 2, Y + 2, Z + 2)
+        self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
+        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        count = torch.empty((B, X), dtype=torch.int32, device=self.device)
+        xinit = self._xinit()
+        st = self._lib.tf_setup_multiphase(_lib.ptr(cls_t), _lib.ptr(cond), self._n_classes,
+                                           _lib.ptr(xinit), _lib.ptr(self._codes),
+                                           _lib.ptr(self._fields[0]), _lib.ptr(self._fields[1]),
+                                           _lib.ptr(count), B, X, Y, Z, self._stream())
+        _lib.check(st, "tf_setup_multiphase")
+        self.vol_x = count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+
+    def _class_image(self, labels, cls_of_label) -> np.ndarray:
+        img = self.cpu_img
+        if labels.size and labels.dtype.kind in "iubf" and labels.min() >= 0 and labels.max() < 65536 \
+                and np.array_equal(labels, np.floor(labels)):
+            lut = np.zeros(int(labels.max()) + 1, dtype=np.uint8)
+            lut[labels.astype(np.int64)] = cls_of_label
+            return np.ascontiguousarray(lut[img.astype(np.int64, copy=False)])
+        idx = np.searchsorted(labels, img)
+        return np.ascontiguousarray(cls_of_label[idx])
+
+    def return_mask(self, img):
+        if len(self.conductive_labels) == 0:
+            return torch.zeros_like(img)
+        conductive = torch.tensor(self.conductive_labels, dtype=self.precision, device=self.device)
+        return torch.isin(img, conductive).to(self.precision)
+
+    # ---- reference attributes, derived on demand from the class codes ----
+    def _face_maps(self):
+        c = self._codes.long()
+        hm = self._hm_lut
+        n = self._n_classes
+        mid = c[:, :, 1:-1, 1:-1]
+        D_x = hm.view(-1)[mid[:, :-1] * n + mid[:, 1:]]
+        inner = c[:, 1:-1]
+        D_y = hm.view(-1)[inner[:, :, :-1, 1:-1] * n + inner[:, :, 1:, 1:-1]]
+        D_z = hm.view(-1)[inner[:, :, 1:-1, :-1] * n + inner[:, :, 1:-1, 1:]]
+        return D_x, D_y, D_z
+
+    @property
+    def D_x(self):
+        return self._face_maps()[0]
+
+    @property
+    def D_y(self):
+        return self._face_maps()[1]
+
+    @property
+    def D_z(self):
+        return self._face_maps()[2]
+
+    @property
+    def factor(self) -> torch.Tensor:
+        """Sum of face conductances, Dirichlet faces doubled, inf where 0
+        (taufactor/taufactor.py:598-604)."""
+        D_x, D_y, D_z = self._face_maps()
+        factor = D_x[:, :-1] + D_x[:, 1:] + D_y[:, :, :-1] + D_y[:, :, 1:] + \
+            D_z[:, :, :, :-1] + D_z[:, :, :, 1:]
+        factor[:, 0] += D_x[:, 0]
+        factor[:, -1] += D_x[:, -1]
+        factor[factor == 0] = torch.inf
+        return factor
+
+    def vertical_flux(self):
+        """taufactor/taufactor.py:615-620"""
+        f = self.field
+        return self.D_x[:, 1:-1] * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])

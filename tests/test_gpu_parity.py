@@ -1,0 +1,160 @@
+"""GPU parity: the HIP path (through libtauhip's C-ABI) against the CPU oracle.
+
+Iterates are compared value for value (np.array_equal, which also treats
++0 and -0 as equal: the reference leaves inactive voxels at a signed zero
+whose sign carries no information).  Per-layer float64 sums are compared to
+the oracle's float64 sums of the same float32 values (only the summation
+order differs: rtol 1e-12).  Full solves must stop at the same iteration
+with the same tau and D_eff (both decide in float64; the public
+arrays are float32, hence rtol 2e-7 and 1e-6)."""
+import numpy as np
+import pytest
+import torch
+
+import taufactor_amd as tau
+from oracle.restate import OracleMultiPhaseSolver, OracleSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _binary(shape, p=0.55, seed=0):
+    rng = np.random.default_rng(seed)
+    return (rng.random(shape) < p).astype(np.uint8)
+
+
+def _labels(shape, n=3, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, n, size=shape)
+
+
+def _field(s):
+    return s.field.cpu().numpy()
+
+
+SHAPES = [(5, 7, 9), (33, 17, 70), (20, 21, 21), (2, 1, 3), (1, 6, 5)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("H", [1, 2, 3, 4, 5, 8])
+def test_solver_iterates_bit_identical(shape, H):
+    img = _binary(shape, seed=sum(shape) + H)
+    k = 23
+    s = tau.Solver(img)
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=k, verbose=False)
+    o = OracleSolver(img).run(k)
+    assert s.iter == k
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+@pytest.mark.parametrize("tile", [(1, 1, 1), (3, 5, 4), (7, 64, 9), (16, 16, 2)])
+@pytest.mark.parametrize("H", [2, 4, 7])
+def test_solver_iterates_with_small_tiles(tile, H):
+    img = _binary((19, 23, 37), seed=5)
+    s = tau.Solver(img)
+    s.half_sweeps_per_pass = H
+    s.tile = tile
+    s.solve(iter_limit=3, verbose=False)        # passes then start on colour 1
+    s.solve(iter_limit=3 + 31, verbose=False)
+    o = OracleSolver(img).run(34)
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+def test_solver_batch_iterates():
+    img = np.stack([_binary((12, 10, 14), seed=i) for i in range(3)])
+    s = tau.Solver(img)
+    s.solve(iter_limit=41, verbose=False)
+    o = OracleSolver(img).run(41)
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+def test_isolated_voxels_and_decay():
+    img = np.zeros((6, 7, 8), dtype=np.uint8)
+    img[2, 3, 3] = 1          # isolated conductive voxel: decays as (1-w)^k
+    img[:, 1, 1] = 1          # straight channel
+    img[0, 5, 5] = 1          # touches only the x=0 ghost
+    s = tau.Solver(img)
+    s.solve(iter_limit=57, verbose=False)
+    o = OracleSolver(img).run(57)
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+@pytest.mark.parametrize("shape", [(5, 7, 9), (33, 17, 70), (20, 21, 21), (1, 6, 5)])
+@pytest.mark.parametrize("H", [1, 2, 4, 6])
+def test_multiphase_iterates_bit_identical(shape, H):
+    img = _labels(shape, n=4, seed=sum(shape))
+    Ds = {0: 0.0, 1: 1.0, 2: 0.1, 3: 2.5}
+    s = tau.MultiPhaseSolver(img, dict(Ds))
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=27, verbose=False)
+    o = OracleMultiPhaseSolver(img, dict(Ds)).run(27)
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+def test_multiphase_label_zero_conductive_and_odd_values():
+    img = _labels((14, 15, 16), n=3, seed=9)
+    Ds = {0: 0.7, 1: 0.0, 2: 1.0 / 3.0}
+    s = tau.MultiPhaseSolver(img, dict(Ds))
+    s.tile = (5, 6, 7)
+    s.solve(iter_limit=33, verbose=False)
+    o = OracleMultiPhaseSolver(img, dict(Ds)).run(33)
+    assert np.array_equal(_field(s), o.field.numpy())
+
+
+def test_inplace_half_sweep_kernel_matches_oracle():
+    from taufactor_amd import _lib
+
+    img = _binary((9, 11, 13), seed=2)
+    s = tau.Solver(img)
+    lib = _lib.load()
+    f = s.field
+    for it in range(7):
+        st = lib.tf_sweep_half_inplace(_lib.ptr(f), _lib.ptr(s._codes), None, 0, _lib.KIND_SOLVER,
+                                       1, 9, 11, 13, s._w32, it & 1, _lib.stream_handle(s.device))
+        _lib.check(st, "tf_sweep_half_inplace")
+    o = OracleSolver(img).run(7)
+    assert np.array_equal(f.cpu().numpy(), o.field.numpy())
+
+
+@pytest.mark.parametrize("kind", ["solver", "multiphase"])
+def test_layer_sums_match_float64_oracle(kind):
+    shape = (17, 19, 23)
+    if kind == "solver":
+        img = _binary(shape, seed=3)
+        s, o = tau.Solver(img), OracleSolver(img)
+    else:
+        img = _labels(shape, seed=3)
+        Ds = {0: 0.0, 1: 1.0, 2: 0.25}
+        s, o = tau.MultiPhaseSolver(img, dict(Ds)), OracleMultiPhaseSolver(img, dict(Ds))
+    s.solve(iter_limit=60, verbose=False)
+    o.run(60)
+    flux, conc = s._layer_sums()
+    vf = o.vertical_flux().to(torch.float64).sum((2, 3)).numpy()
+    cs = o.field[:, 1:-1, 1:-1, 1:-1].to(torch.float64).sum((2, 3)).numpy()
+    np.testing.assert_allclose(flux, vf, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(conc, cs, rtol=1e-12, atol=1e-12)
+    f2, c2 = s._layer_sums()
+    assert np.array_equal(flux, f2) and np.array_equal(conc, c2)  # deterministic
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_solver_full_solve_matches_oracle(seed):
+    img = _binary((40, 32, 36), p=0.6, seed=seed)
+    s = tau.Solver(img)
+    s.solve(verbose=False)
+    o = OracleSolver(img).solve()
+    assert s.iter == o.iter and s.converged == o.converged
+    assert np.array_equal(_field(s), o.field.numpy())
+    np.testing.assert_allclose(s.tau, o.tau, rtol=2e-7)
+    np.testing.assert_allclose(s.D_eff, o.D_eff, rtol=1e-6)
+
+
+def test_multiphase_full_solve_matches_oracle():
+    img = _labels((36, 30, 28), n=3, seed=4)
+    Ds = {0: 0.0, 1: 1.0, 2: 0.1}
+    s = tau.MultiPhaseSolver(img, dict(Ds))
+    s.solve(verbose=False, conv_crit=1e-3)
+    o = OracleMultiPhaseSolver(img, dict(Ds)).solve(conv_crit=1e-3)
+    assert s.iter == o.iter
+    assert np.array_equal(_field(s), o.field.numpy())
+    np.testing.assert_allclose(s.tau, o.tau, rtol=2e-7)
