@@ -41,10 +41,13 @@ extern "C" {
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
 
-/* Tile shape of the temporally blocked sweep (interior voxels per workgroup
- * in y, z and x).  Zero fields select the built-in defaults. */
+/* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
+ * tile of (waves*rows_per_wave - 2*n_half) rows in y by (128 - 2*n_half)
+ * voxels in z along tx planes of x; one wavefront per row.  Compiled shapes
+ * (waves x rows_per_wave): 8x2, 8x3, 16x1, 4x4.  Zero fields select the
+ * defaults (8x2 for n_half <= 3, 8x3 above; tx 64). */
 typedef struct tf_tile {
-    int ty, tz, tx;
+    int waves, rows_per_wave, tx;
 } tf_tile;
 
 /* ABI version of the loaded library (TF_ABI_VERSION). No reference
@@ -94,7 +97,7 @@ int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
  *   omega    the over-relaxation factor rounded to float32
  *   n_half   1..TF_MAX_FUSED_HALF_SWEEPS
  *   tile     NULL for the default tile shape                              */
-#define TF_MAX_FUSED_HALF_SWEEPS 8
+#define TF_MAX_FUSED_HALF_SWEEPS 6
 int tf_sweep(const float* src, float* dst, const uint8_t* codes,
              const float* hm_lut, int n_classes, int kind,
              int B, int X, int Y, int Z, float omega, int first_colour,

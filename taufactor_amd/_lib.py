@@ -15,7 +15,7 @@ ABI_VERSION = 1
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
-MAX_FUSED_HALF_SWEEPS = 8
+MAX_FUSED_HALF_SWEEPS = 6
 
 _c_int = ctypes.c_int
 _c_float = ctypes.c_float
@@ -23,7 +23,7 @@ _vp = ctypes.c_void_p
 
 
 class Tile(ctypes.Structure):
-    _fields_ = [("ty", ctypes.c_int), ("tz", ctypes.c_int), ("tx", ctypes.c_int)]
+    _fields_ = [("waves", ctypes.c_int), ("rows_per_wave", ctypes.c_int), ("tx", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); the exported surface of include/tauhip.h

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "tauhip.h"
@@ -338,7 +339,7 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
 // temporally blocked sweep: H half-sweeps per HBM pass
 // ---------------------------------------------------------------------------
 // A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
-// coordinates) and streams it plane by plane along x through a ring of H+2
+// coordinates) and streams it plane by plane along x through a ring of H+3
 // planes in LDS, each plane extended by H voxels in y and z.  Half-sweep h
 // (h = 0..H-1, colour first_colour^(h&1)) runs on plane s-h at step s, on
 // the tile grown by H-1-h voxels, so that after the last half-sweep the
@@ -347,11 +348,18 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
 // so reading src and writing dst (no in-place hazard between workgroups
 // that recompute each other's halos) gives the in-place result.
 //
+// Mapping: one wavefront per tile row (y), one lane per pair of voxels in z,
+// so a row is 128 voxels of z (tile interior TZ = 128 - 2H) and the colour
+// pattern of a row is uniform across the wave.  An LDS row stores the 64
+// even-z voxels, then the 64 odd-z voxels: every LDS access of a half-sweep
+// is lane-contiguous (no bank conflicts) at a constant offset from one base.
+// Each wave owns RPW rows (interleaved), NW waves make EY = NW*RPW rows
+// (tile interior TY = EY - 2H).  The next plane is prefetched into registers
+// while the half-sweeps of the current step run.
+//
 // Workgroups are numbered so that consecutive tiles land on the same XCD
 // (hardware dispatch sends workgroup i to XCD i mod 8): neighbouring tiles
 // then share their halo lines in that XCD's L2.
-constexpr int kSweepThreads = 256;
-
 struct SweepArgs {
     const float* src;
     float* dst;
@@ -361,21 +369,41 @@ struct SweepArgs {
     Vol v;
     float w;
     int colour0;
-    int TY, TZ, TX;
+    int TX;
     int nTz, nTy, nTx, total;
 };
 
-template <int H>
-__host__ __device__ constexpr int ring_planes() { return H + 2; }
+// compile-time loop: f(std::integral_constant<int, I>{}) for I in [B, E)
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
-template <int H, int KIND>
-__global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
+constexpr int kPrefetch = 3;           // planes in flight per workgroup
+constexpr int kRowPairs = 64;          // lanes per row
+constexpr int kRowLen = 2 * kRowPairs;  // voxels per LDS row
+
+template <int H, int NW, int RPW>
+struct SweepShape {
+    static constexpr int EY = NW * RPW;
+    static constexpr int TY = EY - 2 * H;
+    static constexpr int TZ = kRowLen - 2 * H;
+    static constexpr int NP = H + 3;      // planes in the LDS ring
+    static constexpr int PS = EY * kRowLen;  // floats (and code bytes) per plane
+};
+
+template <int H, int KIND, int NW, int RPW>
+__global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
+    using Sh = SweepShape<H, NW, RPW>;
+    constexpr int EY = Sh::EY, TY = Sh::TY, TZ = Sh::TZ, NP = Sh::NP, PS = Sh::PS;
+    static_assert(TY >= 1, "tile has no interior rows");
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int NP = ring_planes<H>();
-    const int EY = a.TY + 2 * H, EZ = a.TZ + 2 * H, EP = EY * EZ;
     float* sF = reinterpret_cast<float*>(smem);
-    uint8_t* sC = reinterpret_cast<uint8_t*>(sF + NP * EP);
-    float* sT = reinterpret_cast<float*>(sC + ((NP * EP + 15) & ~15));
+    uint8_t* sC = reinterpret_cast<uint8_t*>(sF + NP * PS);
+    float* sT = reinterpret_cast<float*>(sC + NP * PS);
 
     // XCD-aware tile numbering (grid is a multiple of 8)
     const int per = gridDim.x >> 3;
@@ -392,160 +420,242 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweep(SweepArgs a) {
     float* __restrict__ D = a.dst + (size_t)b * v.sB;
     const uint8_t* __restrict__ Cg = a.codes + (size_t)b * v.sB;
 
-    const int y0 = 1 + ty * a.TY, z0 = 1 + tz * a.TZ, x0 = 1 + tx * a.TX;
-    const int y1 = min(y0 + a.TY, v.Y + 1), z1 = min(z0 + a.TZ, v.Z + 1), x1 = min(x0 + a.TX, v.X + 1);
-    const int ey0 = y0 - H, ez0 = z0 - H;  // padded coordinate of LDS (0,0)
-    const int ly0 = max(ey0, 0), ly1 = min(y1 + H, v.Y + 2);
-    const int lz0 = max(ez0, 0), lz1 = min(z1 + H, v.Z + 2);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * a.TX;
+    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + a.TX, v.X + 1);
+    const int ey0 = y0 - H, ez0 = z0 - H;
+    const int zl = ez0 + 2 * lane;  // z of this lane's even element
+    const bool zok0 = zl >= 0 && zl <= v.Z + 1;
+    const bool zok1 = zl + 1 >= 0 && zl + 1 <= v.Z + 1;
     const int lx1 = min(x1 + H, v.X + 2);
-    const int nly = ly1 - ly0, nlz = lz1 - lz0;
-    const int tid = threadIdx.x;
 
-    // lookup tables
-    if (KIND == TF_KIND_MULTIPHASE) {
-        for (int i = tid; i < a.ncls * a.ncls; i += kSweepThreads) sT[i] = a.hm[i];
+    if (KIND == TF_KIND_SOLVER) {
+        if (threadIdx.x < 16) sT[threadIdx.x] = threadIdx.x < 9 ? c_rcp[threadIdx.x] : 0.0f;
+    } else {
+        for (int i = threadIdx.x; i < a.ncls * a.ncls; i += NW * 64) sT[i] = a.hm[i];
     }
 
-    const int nload = nly * nlz;
-    const float rinv_load = 1.0f / (float)nlz;
-    auto load_plane = [&](int p) {
-        float* dF = sF + (p % NP) * EP;
-        uint8_t* dC = sC + (p % NP) * EP;
-        const int gbase = p * v.sX;
-        for (int i = tid; i < nload; i += kSweepThreads) {
-            int r = (int)((float)i * rinv_load);
-            if (r * nlz > i) --r;
-            else if ((r + 1) * nlz <= i) ++r;
-            const int zz = lz0 + (i - r * nlz);
-            const int yy = ly0 + r;
-            const int g = gbase + yy * v.sY + zz;
-            const int l = (yy - ey0) * EZ + (zz - ez0);
-            dF[l] = S[g];
-            dC[l] = Cg[g];
+    // ---- plane loads: global -> registers (kPrefetch planes in flight) -> LDS ----
+    // Register slot of plane q is (q - S0) mod kPrefetch; every use below has
+    // a compile-time slot, so the ring stays in registers.
+    float pf[kPrefetch][RPW][2];
+    uint32_t pc[kPrefetch][RPW][2];
+    auto issue_load = [&](int p, auto slot_c) {
+        constexpr int sl = decltype(slot_c)::value;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int y = ey0 + wave + j * NW;
+            const bool rok = y >= 0 && y <= v.Y + 1;
+            const int g = p * v.sX + y * v.sY + zl;
+            pf[sl][j][0] = (rok && zok0) ? S[g] : 0.0f;
+            pf[sl][j][1] = (rok && zok1) ? S[g + 1] : 0.0f;
+            pc[sl][j][0] = (rok && zok0) ? Cg[g] : 0u;
+            pc[sl][j][1] = (rok && zok1) ? Cg[g + 1] : 0u;
+        }
+    };
+    auto commit_load = [&](int p, auto slot_c) {
+        constexpr int sl = decltype(slot_c)::value;
+        float* P = sF + (p % NP) * PS;
+        uint8_t* CP = sC + (p % NP) * PS;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int o = (wave + j * NW) * kRowLen + lane;
+            P[o] = pf[sl][j][0];
+            P[o + kRowPairs] = pf[sl][j][1];
+            CP[o] = (uint8_t)pc[sl][j][0];
+            CP[o + kRowPairs] = (uint8_t)pc[sl][j][1];
+        }
+    };
+
+    // ---- one half-sweep of one plane ----
+    auto stage = [&](int h, int p) {
+        const int grow = H - 1 - h;
+        if (p < max(x0 - grow, 1) || p >= min(x1 + grow, v.X + 1)) return;
+        const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
+        const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
+        const int colour = a.colour0 ^ (h & 1);
+        float* P = sF + (p % NP) * PS;
+        const float* Pm = sF + ((p - 1) % NP) * PS;
+        const float* Pp = sF + ((p + 1) % NP) * PS;
+        const uint8_t* CP = sC + (p % NP) * PS;
+        const uint8_t* CPm = sC + ((p - 1) % NP) * PS;
+        const uint8_t* CPp = sC + ((p + 1) % NP) * PS;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int r = wave + j * NW;
+            const int y = ey0 + r;
+            if (y < ry0 || y >= ry1) continue;
+            // colour of (p,y,z) is (p+y+z+1)&1; the active element of the pair
+            const int act = ((p + y + ez0 + 1) & 1) ^ colour;
+            const int z = zl + act;
+            const bool ok = z >= rz0 && z < rz1;
+            const int o = r * kRowLen + act * kRowPairs + lane;        // own
+            const int oo = r * kRowLen + (act ^ 1) * kRowPairs + lane;  // other element, same pair
+            const int on = act ? oo + 1 : oo - 1;                       // other element, next pair
+            Nbr n;
+            n.c = P[o];
+            n.xm = Pm[o];
+            n.xp = Pp[o];
+            n.ym = P[o - kRowLen];
+            n.yp = P[o + kRowLen];
+            if (act) {
+                n.zm = P[oo];
+                n.zp = P[on];
+            } else {
+                n.zp = P[oo];
+                n.zm = P[on];
+            }
+            float out;
+            if (KIND == TF_KIND_SOLVER) {
+                const int code = CP[o];
+                const float r_ = sT[code];
+                const float d_ = code ? (float)code : __builtin_inff();
+                float s = n.xp + n.xm;
+                s = s + n.yp;
+                s = s + n.ym;
+                s = s + n.zp;
+                s = s + n.zm;
+                s = div_by_count(s, r_, d_, code);
+                s = s - n.c;
+                s = s * a.w;
+                out = n.c + s;
+            } else {
+                const int l = CP[o];
+                const float* row = sT + l * a.ncls;
+                const int lzm = act ? CP[oo] : CP[on];
+                const int lzp = act ? CP[on] : CP[oo];
+                out = update_multiphase(n, row[CPp[o]], row[CPm[o]], row[CP[o + kRowLen]],
+                                        row[CP[o - kRowLen]], row[lzp], row[lzm], p == 1, p == v.X,
+                                        a.w);
+            }
+            if (ok) P[o] = out;
+        }
+    };
+
+    auto store_plane = [&](int q) {
+        if (q < x0 || q >= x1) return;
+        const float* P = sF + (q % NP) * PS;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int r = wave + j * NW;
+            const int y = ey0 + r;
+            if (y < y0 || y >= y1) continue;
+            const int g = q * v.sX + y * v.sY + zl;
+            const int o = r * kRowLen + lane;
+            if (zl >= z0 && zl < z1) D[g] = P[o];
+            if (zl + 1 >= z0 && zl + 1 < z1) D[g + 1] = P[o + kRowPairs];
         }
     };
 
     const int S0 = max(x0 - (H - 1), 1);
     const int S1 = (x1 - 1) + (H - 1);
-    load_plane(S0 - 1);
-    if (S0 < lx1) load_plane(S0);
+    using I0 = std::integral_constant<int, 0>;
+    // planes S0-1 .. S0+1 go straight to LDS; S0+2 .. S0+kPrefetch are put in flight
+    issue_load(S0 - 1, I0{});
+    commit_load(S0 - 1, I0{});
+    if (S0 < lx1) {
+        issue_load(S0, I0{});
+        commit_load(S0, I0{});
+    }
+    if (S0 + 1 < lx1) {
+        issue_load(S0 + 1, I0{});
+        commit_load(S0 + 1, I0{});
+    }
+    static_for<2, kPrefetch + 2>([&](auto d_c) {
+        constexpr int d = decltype(d_c)::value;
+        if (S0 + d < lx1) issue_load(S0 + d, std::integral_constant<int, d % kPrefetch>{});
+    });
+    __syncthreads();
 
-    for (int s = S0; s <= S1; ++s) {
-        if (s + 1 < lx1) load_plane(s + 1);
-        __syncthreads();
+    // step s: put plane s+1+kPrefetch in flight, run half-sweep h on plane
+    // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS
+    for (int s0 = S0; s0 <= S1; s0 += kPrefetch) {
+        static_for<0, kPrefetch>([&](auto u_c) {
+            constexpr int u = decltype(u_c)::value;
+            const int s = s0 + u;
+            if (s > S1) return;
+            if (s + 1 + kPrefetch < lx1)
+                issue_load(s + 1 + kPrefetch, std::integral_constant<int, (u + 1) % kPrefetch>{});
 #pragma unroll
-        for (int h = 0; h < H; ++h) {
-            const int p = s - h;
-            const int grow = H - 1 - h;
-            if (p >= max(x0 - grow, 1) && p < min(x1 + grow, v.X + 1)) {
-                const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
-                const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
-                const int nry = ry1 - ry0, nrz = rz1 - rz0;
-                const int npair = (nrz + 1) >> 1;
-                const int nitem = nry * npair;
-                const float rinv = 1.0f / (float)npair;
-                const int colour = a.colour0 ^ (h & 1);
-                float* P = sF + (p % NP) * EP;
-                const float* Pm = sF + ((p - 1) % NP) * EP;
-                const float* Pp = sF + ((p + 1) % NP) * EP;
-                const uint8_t* CP = sC + (p % NP) * EP;
-                const uint8_t* CPm = sC + ((p - 1) % NP) * EP;
-                const uint8_t* CPp = sC + ((p + 1) % NP) * EP;
-                for (int i = tid; i < nitem; i += kSweepThreads) {
-                    int r = (int)((float)i * rinv);
-                    if (r * npair > i) --r;
-                    else if ((r + 1) * npair <= i) ++r;
-                    const int k = i - r * npair;
-                    const int yy = ry0 + r;
-                    // interior colour (x-1)+(y-1)+(z-1) == colour  <=>  (p+yy+zz+1)&1 == colour
-                    int zz = rz0 + 2 * k;
-                    zz += ((p + yy + zz + 1) & 1) ^ colour;
-                    if (zz >= rz1) continue;
-                    const int l = (yy - ey0) * EZ + (zz - ez0);
-                    Nbr n;
-                    n.c = P[l];
-                    n.xp = Pp[l];
-                    n.xm = Pm[l];
-                    n.yp = P[l + EZ];
-                    n.ym = P[l - EZ];
-                    n.zp = P[l + 1];
-                    n.zm = P[l - 1];
-                    float out;
-                    if (KIND == TF_KIND_SOLVER) {
-                        out = update_solver(n, CP[l], a.w);
-                    } else {
-                        const float* row = sT + CP[l] * a.ncls;
-                        out = update_multiphase(n, row[CPp[l]], row[CPm[l]], row[CP[l + EZ]],
-                                                row[CP[l - EZ]], row[CP[l + 1]], row[CP[l - 1]],
-                                                p == 1, p == v.X, a.w);
-                    }
-                    P[l] = out;
-                }
+            for (int h = 0; h < H; ++h) {
+                stage(h, s - h);
+                if (h + 1 < H) __syncthreads();
             }
+            store_plane(s - (H - 1));
+            if (s + 2 < lx1) commit_load(s + 2, std::integral_constant<int, (u + 2) % kPrefetch>{});
             __syncthreads();
+        });
+    }
+}
+
+template <int H, int NW, int RPW>
+static size_t sweep_lds_bytes(int kind, int ncls) {
+    using Sh = SweepShape<H, NW, RPW>;
+    size_t bytes = (size_t)Sh::NP * Sh::PS * (sizeof(float) + 1);
+    bytes += kind == TF_KIND_MULTIPHASE ? (size_t)ncls * ncls * sizeof(float) : 16 * sizeof(float);
+    return bytes;
+}
+
+template <int H, int KIND, int NW, int RPW>
+static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
+    using Sh = SweepShape<H, NW, RPW>;
+    if constexpr (Sh::TY < 1) {
+        return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
+    } else {
+        const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
+        if (lds > 160 * 1024)
+            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
+        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
+        a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
+        a.nTx = (a.v.X + a.TX - 1) / a.TX;
+        long long total = (long long)a.nTz * a.nTy * a.nTx * B;
+        if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
+        a.total = (int)total;
+        const int grid = (int)(((total + 7) / 8) * 8);
+        static bool attr_set = false;
+        if (!attr_set && lds > 64 * 1024) {
+            // opt in to more than 64 KiB of dynamic LDS; if the runtime
+            // refuses, the launch below reports the error
+            if (hipFuncSetAttribute((const void*)k_sweep<H, KIND, NW, RPW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                (void)hipGetLastError();
+            attr_set = true;
         }
-        // plane s-H+1 has had all H half-sweeps; write its owned tile
-        const int q = s - (H - 1);
-        if (q >= x0 && q < x1) {
-            const float* P = sF + (q % NP) * EP;
-            const int oy = y1 - y0, oz = z1 - z0;
-            const int nst = oy * oz;
-            const float rinv = 1.0f / (float)oz;
-            for (int i = tid; i < nst; i += kSweepThreads) {
-                int r = (int)((float)i * rinv);
-                if (r * oz > i) --r;
-                else if ((r + 1) * oz <= i) ++r;
-                const int zz = z0 + (i - r * oz);
-                const int yy = y0 + r;
-                D[q * v.sX + yy * v.sY + zz] = P[(yy - ey0) * EZ + (zz - ez0)];
-            }
-        }
+        hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        TF_LAUNCH_CHECK("k_sweep");
+        return 0;
+    }
+}
+
+// workgroup shapes (waves x rows per wave) that are compiled
+template <int H, int KIND>
+static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
+    if (nw == 8 && rpw == 2) return launch_sweep<H, KIND, 8, 2>(a, B, st);
+    if (nw == 8 && rpw == 3) return launch_sweep<H, KIND, 8, 3>(a, B, st);
+    if (nw == 16 && rpw == 1) return launch_sweep<H, KIND, 16, 1>(a, B, st);
+    if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled (use 8x2, 8x3, 16x1, 4x4)",
+                nw, rpw);
+}
+
+template <int KIND>
+static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
+    switch (H) {
+        case 1: return dispatch_shape<1, KIND>(a, B, nw, rpw, st);
+        case 2: return dispatch_shape<2, KIND>(a, B, nw, rpw, st);
+        case 3: return dispatch_shape<3, KIND>(a, B, nw, rpw, st);
+        case 4: return dispatch_shape<4, KIND>(a, B, nw, rpw, st);
+        case 5: return dispatch_shape<5, KIND>(a, B, nw, rpw, st);
+        case 6: return dispatch_shape<6, KIND>(a, B, nw, rpw, st);
+        default: return fail(-1, "n_half=%d out of range 1..%d", H, TF_MAX_FUSED_HALF_SWEEPS);
     }
 }
 
 static void default_tile(int H, tf_tile* t) {
-    t->tz = 64;
-    t->ty = H <= 2 ? 16 : (H <= 4 ? 16 : 8);
+    t->waves = 8;
+    t->rows_per_wave = H <= 3 ? 2 : 3;
     t->tx = 64;
-}
-
-static size_t sweep_lds_bytes(int H, int kind, int ncls, const tf_tile& t) {
-    size_t EP = (size_t)(t.ty + 2 * H) * (t.tz + 2 * H);
-    size_t NP = (size_t)H + 2;
-    size_t bytes = NP * EP * sizeof(float) + ((NP * EP + 15) & ~(size_t)15);
-    if (kind == TF_KIND_MULTIPHASE) bytes += (size_t)ncls * ncls * sizeof(float);
-    return bytes;
-}
-
-template <int H, int KIND>
-static int launch_sweep(const SweepArgs& a, size_t lds, int grid, hipStream_t st) {
-    static bool attr_set = false;
-    if (!attr_set && lds > 64 * 1024) {
-        // opt in to more than 64 KiB of dynamic LDS; if the runtime refuses,
-        // the launch below reports the error
-        if (hipFuncSetAttribute((const void*)k_sweep<H, KIND>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
-            (void)hipGetLastError();
-        attr_set = true;
-    }
-    hipLaunchKernelGGL((k_sweep<H, KIND>), dim3(grid), dim3(kSweepThreads), lds, st, a);
-    TF_LAUNCH_CHECK("k_sweep");
-    return 0;
-}
-
-template <int KIND>
-static int dispatch_sweep(int H, const SweepArgs& a, size_t lds, int grid, hipStream_t st) {
-    switch (H) {
-        case 1: return launch_sweep<1, KIND>(a, lds, grid, st);
-        case 2: return launch_sweep<2, KIND>(a, lds, grid, st);
-        case 3: return launch_sweep<3, KIND>(a, lds, grid, st);
-        case 4: return launch_sweep<4, KIND>(a, lds, grid, st);
-        case 5: return launch_sweep<5, KIND>(a, lds, grid, st);
-        case 6: return launch_sweep<6, KIND>(a, lds, grid, st);
-        case 7: return launch_sweep<7, KIND>(a, lds, grid, st);
-        case 8: return launch_sweep<8, KIND>(a, lds, grid, st);
-        default: return fail(-1, "n_half=%d out of range 1..%d", H, TF_MAX_FUSED_HALF_SWEEPS);
-    }
 }
 
 extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, const float* hm_lut,
@@ -563,32 +673,21 @@ extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, cons
     tf_tile t;
     default_tile(n_half, &t);
     if (tile) {
-        if (tile->ty > 0) t.ty = tile->ty;
-        if (tile->tz > 0) t.tz = tile->tz;
+        if (tile->waves > 0) t.waves = tile->waves;
+        if (tile->rows_per_wave > 0) t.rows_per_wave = tile->rows_per_wave;
         if (tile->tx > 0) t.tx = tile->tx;
     }
-    if (t.ty < 1 || t.tz < 1 || t.tx < 1 || t.ty > 256 || t.tz > 1024 || t.tx > 4096)
-        return fail(-1, "invalid tile %dx%dx%d", t.ty, t.tz, t.tx);
-    size_t lds = sweep_lds_bytes(n_half, kind, n_classes, t);
-    if (lds > 160 * 1024)
-        return fail(-1, "tile %dx%dx%d with n_half=%d needs %zu B of LDS (> 160 KiB)", t.ty, t.tz,
-                    t.tx, n_half, lds);
+    if (t.tx < 1 || t.tx > 65536) return fail(-1, "invalid tile tx=%d", t.tx);
     SweepArgs a;
     a.src = src; a.dst = dst; a.codes = codes; a.hm = hm_lut; a.ncls = n_classes;
     a.v = make_vol(X, Y, Z);
     a.w = omega;
     a.colour0 = first_colour;
-    a.TY = t.ty; a.TZ = t.tz; a.TX = t.tx;
-    a.nTz = (Z + t.tz - 1) / t.tz;
-    a.nTy = (Y + t.ty - 1) / t.ty;
-    a.nTx = (X + t.tx - 1) / t.tx;
-    long long total = (long long)a.nTz * a.nTy * a.nTx * B;
-    if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
-    a.total = (int)total;
-    int grid = (int)(((total + 7) / 8) * 8);
+    a.TX = t.tx;
     hipStream_t st = (hipStream_t)stream;
-    if (kind == TF_KIND_SOLVER) return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, lds, grid, st);
-    return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, lds, grid, st);
+    if (kind == TF_KIND_SOLVER)
+        return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, B, t.waves, t.rows_per_wave, st);
+    return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,7 +696,7 @@ extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, cons
 // One block per (x, b).  Each thread accumulates a fixed strided subset of
 // the layer in float64; the block then reduces in a fixed tree, so the result
 // does not depend on scheduling.
-constexpr int kSumThreads = 256;
+constexpr int kSumThreads = 1024;
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -69,16 +69,18 @@ class SORSolver(ABC):
 
         self._lib = _lib.load()
         self.half_sweeps_per_pass = DEFAULT_HALF_SWEEPS_PER_PASS
-        self.tile = None  # (ty, tz, tx) or None for the library default
+        self.tile = None  # (waves, rows_per_wave, tx) or None for the library default
 
         self._fields = None  # two padded float32 buffers (ping-pong)
         self._cur = 0
+        self._launch_events = None  # list -> (start, end, n_launches) per _advance
         self._codes = None
         self._hm_lut = None
         self._n_classes = 0
         self._spans_cache = {}
         with torch.cuda.device(self.device):
             self._setup_device()
+            self._run_setup()
             B, X = self.batch_size, self.Nx
             self._flux_sum = torch.zeros((B, max(X - 1, 1)), dtype=torch.float64, device=self.device)
             self._conc_sum = torch.zeros((B, X), dtype=torch.float64, device=self.device)
@@ -93,7 +95,24 @@ class SORSolver(ABC):
     # ---------------- hooks the subclasses implement ----------------
     @abstractmethod
     def _setup_device(self) -> None:
-        """Pack codes, write both field buffers, set self.vol_x."""
+        """Upload the input, allocate codes and field buffers, set self.vol_x."""
+
+    @abstractmethod
+    def _run_setup(self) -> None:
+        """Write codes and both field buffers from the device-resident input."""
+
+    def reset(self) -> None:
+        """Return to the initial field and iteration 0 without touching the
+        host image: the device-resident input is packed again."""
+        with torch.cuda.device(self.device):
+            self._run_setup()
+        self._cur = 0
+        self.converged = False
+        self.old_tau = 0
+        self.iter = 0
+        self.tau = None
+        self.tau_x = None
+        self.D_eff = None
 
     @abstractmethod
     def compute_metrics(self):
@@ -148,6 +167,12 @@ class SORSolver(ABC):
         hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
         it = self.iter
         per = max(1, min(int(self.half_sweeps_per_pass), _lib.MAX_FUSED_HALF_SWEEPS))
+        ev = self._launch_events
+        if ev is not None:
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            launches = 0
         while n_half > 0:
             h = min(per, n_half)
             src = self._fields[self._cur]
@@ -159,6 +184,11 @@ class SORSolver(ABC):
             self._cur = 1 - self._cur
             it += h
             n_half -= h
+            if ev is not None:
+                launches += 1
+        if ev is not None:
+            t1.record()
+            ev.append((t0, t1, launches))
 
     def _layer_sums(self):
         """Per-layer float64 flux and concentration sums, copied to the host."""
@@ -400,17 +430,22 @@ class Solver(ThroughTransportSolver):
 
     def _setup_device(self):
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
-        mask = torch.from_numpy(np.ascontiguousarray(self.cpu_img != 0, dtype=np.uint8)).to(self.device)
+        self._input_dev = torch.from_numpy(
+            np.ascontiguousarray(self.cpu_img != 0, dtype=np.uint8)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
         self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
-        count = torch.empty((B, X), dtype=torch.int32, device=self.device)
-        xinit = self._xinit()
-        st = self._lib.tf_setup_solver(_lib.ptr(mask), _lib.ptr(xinit), _lib.ptr(self._codes),
-                                       _lib.ptr(self._fields[0]), _lib.ptr(self._fields[1]),
-                                       _lib.ptr(count), B, X, Y, Z, self._stream())
+        self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
+        self._xinit_dev = self._xinit()
+
+    def _run_setup(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        st = self._lib.tf_setup_solver(_lib.ptr(self._input_dev), _lib.ptr(self._xinit_dev),
+                                       _lib.ptr(self._codes), _lib.ptr(self._fields[0]),
+                                       _lib.ptr(self._fields[1]), _lib.ptr(self._count),
+                                       B, X, Y, Z, self._stream())
         _lib.check(st, "tf_setup_solver")
-        self.vol_x = count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+        self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     @property
     def factor(self) -> torch.Tensor:
@@ -500,20 +535,24 @@ class MultiPhaseSolver(ThroughTransportSolver):
         self._d_cls = d_cls
         self._n_classes = len(d_cls)
         cls_img = self._class_image(labels, cls_of_label)
-        cls_t = torch.from_numpy(cls_img).to(self.device)
+        self._input_dev = torch.from_numpy(cls_img).to(self.device)
         self._hm_lut = torch.from_numpy(np.ascontiguousarray(hm)).to(self.device)
-        cond = torch.from_numpy((d_cls > 0).astype(np.uint8)).to(self.device)
+        self._cls_cond = torch.from_numpy((d_cls > 0).astype(np.uint8)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
         self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
-        count = torch.empty((B, X), dtype=torch.int32, device=self.device)
-        xinit = self._xinit()
-        st = self._lib.tf_setup_multiphase(_lib.ptr(cls_t), _lib.ptr(cond), self._n_classes,
-                                           _lib.ptr(xinit), _lib.ptr(self._codes),
-                                           _lib.ptr(self._fields[0]), _lib.ptr(self._fields[1]),
-                                           _lib.ptr(count), B, X, Y, Z, self._stream())
+        self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
+        self._xinit_dev = self._xinit()
+
+    def _run_setup(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        st = self._lib.tf_setup_multiphase(_lib.ptr(self._input_dev), _lib.ptr(self._cls_cond),
+                                           self._n_classes, _lib.ptr(self._xinit_dev),
+                                           _lib.ptr(self._codes), _lib.ptr(self._fields[0]),
+                                           _lib.ptr(self._fields[1]), _lib.ptr(self._count),
+                                           B, X, Y, Z, self._stream())
         _lib.check(st, "tf_setup_multiphase")
-        self.vol_x = count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+        self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     def _class_image(self, labels, cls_of_label) -> np.ndarray:
         img = self.cpu_img

@@ -35,7 +35,7 @@ SHAPES = [(5, 7, 9), (33, 17, 70), (20, 21, 21), (2, 1, 3), (1, 6, 5)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-@pytest.mark.parametrize("H", [1, 2, 3, 4, 5, 8])
+@pytest.mark.parametrize("H", [1, 2, 3, 4, 5, 6])
 def test_solver_iterates_bit_identical(shape, H):
     img = _binary(shape, seed=sum(shape) + H)
     k = 23
@@ -47,10 +47,10 @@ def test_solver_iterates_bit_identical(shape, H):
     assert np.array_equal(_field(s), o.field.numpy())
 
 
-@pytest.mark.parametrize("tile", [(1, 1, 1), (3, 5, 4), (7, 64, 9), (16, 16, 2)])
-@pytest.mark.parametrize("H", [2, 4, 7])
+@pytest.mark.parametrize("tile", [(8, 2, 1), (16, 1, 5), (4, 4, 9), (8, 3, 2)])
+@pytest.mark.parametrize("H", [2, 3, 4, 6])
 def test_solver_iterates_with_small_tiles(tile, H):
-    img = _binary((19, 23, 37), seed=5)
+    img = _binary((19, 23, 300), seed=5)
     s = tau.Solver(img)
     s.half_sweeps_per_pass = H
     s.tile = tile
@@ -95,7 +95,7 @@ def test_multiphase_label_zero_conductive_and_odd_values():
     img = _labels((14, 15, 16), n=3, seed=9)
     Ds = {0: 0.7, 1: 0.0, 2: 1.0 / 3.0}
     s = tau.MultiPhaseSolver(img, dict(Ds))
-    s.tile = (5, 6, 7)
+    s.tile = (16, 1, 7)
     s.solve(iter_limit=33, verbose=False)
     o = OracleMultiPhaseSolver(img, dict(Ds)).run(33)
     assert np.array_equal(_field(s), o.field.numpy())
