@@ -125,6 +125,65 @@ int tf_layer_sums(const float* field, const uint8_t* codes,
                   int B, int X, int Y, int Z,
                   double* flux_sum, double* conc_sum, void* stream);
 
+/* ---- uneven batch (no reference counterpart: the reference batches only
+ * volumes of one shape and stops when all have converged,
+ * taufactor/taufactor.py:30-31,143; here every volume keeps the solo
+ * Solver semantics: its own omega, check points, old_tau and stopping
+ * iteration) ---- */
+
+/* One two-phase volume of a batch.  Offsets are element offsets into the
+ * packed buffers; layout of each volume as above (B = 1). */
+typedef struct tf_volume {
+    long long offset;   /* first element of the padded volume in field / codes */
+    int X, Y, Z;
+    float omega;        /* over-relaxation factor rounded to float32 */
+    int sum_offset;     /* first entry in the packed layer-sum arrays (X per volume) */
+    int tile_first;     /* first tile of this volume in the batch tile list */
+    int spans;          /* 1 if a 6-connected conductive cluster joins both x faces */
+    int reserved;
+    double D_mean;      /* tau = D_mean / D_rel (float32 mean porosity, widened) */
+} tf_volume;
+
+/* Device-resident convergence state of one volume; initialise with
+ * active = 1 and everything else 0. */
+typedef struct tf_volume_state {
+    int active;         /* still iterating */
+    int converged;      /* stopped by the convergence rule (not the limit) */
+    int iters;          /* reference iterations done at the last check */
+    int cur;            /* 0: field_a holds the final iterate, 1: field_b */
+    double tau, old_tau, D_rel, rel_err;
+} tf_volume_state;
+
+/* Number of workgroup tiles tf_sweep / tf_batch_sweep use for one volume. */
+int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_tile* tile, int* count);
+
+/* n_half half-sweeps on every active volume of a batch, one launch.  All
+ * active volumes read field_a (src_buffer 0) or field_b (1) and write the
+ * other.  tile_vol[t] = volume of tile t (tiles of volume v are
+ * tile_first .. tile_first + tf_sweep_tile_count - 1).  Same arithmetic as
+ * tf_sweep.                                                              */
+int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* codes,
+                   const tf_volume* vols, const int* tile_vol, int n_tiles,
+                   const tf_volume_state* state, int src_buffer, int first_colour,
+                   int n_half, const tf_tile* tile, void* stream);
+
+/* Per-layer float64 sums of every active volume (layer_vol[l] = volume of
+ * packed layer l; layer x of volume v is l = sum_offset + x).  flux_sum[l]
+ * is 0 for the last layer of a volume.  Same order as tf_layer_sums.      */
+int tf_batch_layer_sums(const float* field_a, const float* field_b, const uint8_t* codes,
+                        const tf_volume* vols, const int* layer_vol, int n_layers,
+                        const tf_volume_state* state, int src_buffer,
+                        double* flux_sum, double* conc_sum, void* stream);
+
+/* The convergence check of every active volume, on the device: adds
+ * iters_per_check to iters, evaluates rel. error and tau from the layer
+ * sums and applies the reference's stopping rules
+ * (taufactor/taufactor.py:109-153, 293-331); a volume stops when it
+ * converges or reaches iter_limit, recording which buffer holds its field. */
+int tf_batch_check(const tf_volume* vols, int nvol, const double* flux_sum,
+                   tf_volume_state* state, int src_buffer, int iters_per_check,
+                   int iter_limit, double conv_crit, void* stream);
+
 /* Host-side test: does any 6-connected cluster of `mask` (host uint8
  * [X][Y][Z]) touch both x faces?  Returns 1 or 0, or <0 on error.
  * Replaces the spanning test of extract_through_feature

@@ -5,7 +5,8 @@ The iteration runs in hand-written gfx950 HIP kernels (libtauhip.so, C-ABI in
 include/tauhip.h) loaded through ctypes; PyTorch-ROCm holds the device
 buffers.  There is no CPU code path.
 """
+from .batch import BatchSolver
 from .solver import MultiPhaseSolver, SORSolver, Solver, ThroughTransportSolver
 
-__all__ = ["Solver", "MultiPhaseSolver", "SORSolver", "ThroughTransportSolver"]
+__all__ = ["Solver", "MultiPhaseSolver", "BatchSolver", "SORSolver", "ThroughTransportSolver"]
 __version__ = "0.1.0"

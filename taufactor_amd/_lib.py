@@ -40,7 +40,23 @@ SIGNATURES = {
     "tf_layer_sums": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                _vp, _vp, _vp]),
     "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "tf_sweep_tile_count": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(Tile),
+                                     ctypes.POINTER(ctypes.c_int)]),
+    "tf_batch_sweep": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _c_int, _c_int,
+                                ctypes.POINTER(Tile), _vp]),
+    "tf_batch_layer_sums": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp]),
+    "tf_batch_check": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, ctypes.c_double, _vp]),
 }
+
+# numpy mirrors of the batch structs (layout checked in tests/test_abi.py)
+import numpy as _np  # noqa: E402
+
+VOLUME_DTYPE = _np.dtype([("offset", _np.int64), ("X", _np.int32), ("Y", _np.int32), ("Z", _np.int32),
+                          ("omega", _np.float32), ("sum_offset", _np.int32), ("tile_first", _np.int32),
+                          ("spans", _np.int32), ("reserved", _np.int32), ("D_mean", _np.float64)])
+STATE_DTYPE = _np.dtype([("active", _np.int32), ("converged", _np.int32), ("iters", _np.int32),
+                         ("cur", _np.int32), ("tau", _np.float64), ("old_tau", _np.float64),
+                         ("D_rel", _np.float64), ("rel_err", _np.float64)])
 
 _lock = threading.Lock()
 _lib = None

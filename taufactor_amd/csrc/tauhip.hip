@@ -58,7 +58,7 @@ struct Vol {
     long long sB;    // padded volume size (X+2)(Y+2)(Z+2)
 };
 
-static Vol make_vol(int X, int Y, int Z) {
+__host__ __device__ static inline Vol make_vol(int X, int Y, int Z) {
     Vol v;
     v.X = X; v.Y = Y; v.Z = Z;
     v.sY = Z + 2;
@@ -75,6 +75,26 @@ static int check_dims(int B, int X, int Y, int Z) {
     if (n >= (1LL << 31))
         return fail(-1, "padded volume of %lld voxels exceeds 2^31", n);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// raw buffer access: 32-bit offsets from a uniform base; an offset at or past
+// the record count reads 0 and drops a write
+// ---------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer descriptor on gfx9
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferFlags);
+}
+__device__ __forceinline__ float ld_f32(rsrc_t r, int byte_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ uint32_t ld_u8(rsrc_t r, int byte_off) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, byte_off, 0, 0);
+}
+__device__ __forceinline__ void st_f32(float val, rsrc_t r, int byte_off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), r, byte_off, 0, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -396,7 +416,10 @@ struct SweepShape {
 };
 
 template <int H, int KIND, int NW, int RPW>
-__global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
+__device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
+                                           const uint8_t* __restrict__ Cg,
+                                           const float* __restrict__ hm, int ncls, const Vol v,
+                                           float w, int colour0, int TX, int tz, int ty, int tx) {
     using Sh = SweepShape<H, NW, RPW>;
     constexpr int EY = Sh::EY, TY = Sh::TY, TZ = Sh::TZ, NP = Sh::NP, PS = Sh::PS;
     static_assert(TY >= 1, "tile has no interior rows");
@@ -404,26 +427,12 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     float* sF = reinterpret_cast<float*>(smem);
     uint8_t* sC = reinterpret_cast<uint8_t*>(sF + NP * PS);
     float* sT = reinterpret_cast<float*>(sC + NP * PS);
-
-    // XCD-aware tile numbering (grid is a multiple of 8)
-    const int per = gridDim.x >> 3;
-    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (logical >= a.total) return;
-    int t = logical;
-    const int tz = t % a.nTz; t /= a.nTz;
-    const int ty = t % a.nTy; t /= a.nTy;
-    const int tx = t % a.nTx;
-    const int b = t / a.nTx;
-
-    const Vol v = a.v;
-    const float* __restrict__ S = a.src + (size_t)b * v.sB;
-    float* __restrict__ D = a.dst + (size_t)b * v.sB;
-    const uint8_t* __restrict__ Cg = a.codes + (size_t)b * v.sB;
+    (void)EY;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * a.TX;
-    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + a.TX, v.X + 1);
+    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
+    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
     const int ey0 = y0 - H, ez0 = z0 - H;
     const int zl = ez0 + 2 * lane;  // z of this lane's even element
     const bool zok0 = zl >= 0 && zl <= v.Z + 1;
@@ -433,7 +442,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     if (KIND == TF_KIND_SOLVER) {
         if (threadIdx.x < 16) sT[threadIdx.x] = threadIdx.x < 9 ? c_rcp[threadIdx.x] : 0.0f;
     } else {
-        for (int i = threadIdx.x; i < a.ncls * a.ncls; i += NW * 64) sT[i] = a.hm[i];
+        for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
     }
 
     // ---- plane loads: global -> registers (kPrefetch planes in flight) -> LDS ----
@@ -443,15 +452,20 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     uint32_t pc[kPrefetch][RPW][2];
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
+        // one buffer descriptor per padded row (the row is wave-uniform): z
+        // outside [0, Z+1] and rows outside [0, Y+1] read as 0 and cost no
+        // memory traffic
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int y = ey0 + wave + j * NW;
             const bool rok = y >= 0 && y <= v.Y + 1;
-            const int g = p * v.sX + y * v.sY + zl;
-            pf[sl][j][0] = (rok && zok0) ? S[g] : 0.0f;
-            pf[sl][j][1] = (rok && zok1) ? S[g + 1] : 0.0f;
-            pc[sl][j][0] = (rok && zok0) ? Cg[g] : 0u;
-            pc[sl][j][1] = (rok && zok1) ? Cg[g + 1] : 0u;
+            const size_t row = (size_t)p * v.sX + (size_t)(rok ? y : 0) * v.sY;
+            const rsrc_t rf = make_rsrc(S + row, rok ? (uint32_t)v.sY * 4u : 0u);
+            const rsrc_t rc = make_rsrc(Cg + row, rok ? (uint32_t)v.sY : 0u);
+            pf[sl][j][0] = ld_f32(rf, zl * 4);
+            pf[sl][j][1] = ld_f32(rf, zl * 4 + 4);
+            pc[sl][j][0] = ld_u8(rc, zl);
+            pc[sl][j][1] = ld_u8(rc, zl + 1);
         }
     };
     auto commit_load = [&](int p, auto slot_c) {
@@ -474,7 +488,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
         if (p < max(x0 - grow, 1) || p >= min(x1 + grow, v.X + 1)) return;
         const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
         const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
-        const int colour = a.colour0 ^ (h & 1);
+        const int colour = colour0 ^ (h & 1);
         float* P = sF + (p % NP) * PS;
         const float* Pm = sF + ((p - 1) % NP) * PS;
         const float* Pp = sF + ((p + 1) % NP) * PS;
@@ -518,16 +532,16 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
                 s = s + n.zm;
                 s = div_by_count(s, r_, d_, code);
                 s = s - n.c;
-                s = s * a.w;
+                s = s * w;
                 out = n.c + s;
             } else {
                 const int l = CP[o];
-                const float* row = sT + l * a.ncls;
+                const float* row = sT + l * ncls;
                 const int lzm = act ? CP[oo] : CP[on];
                 const int lzp = act ? CP[on] : CP[oo];
                 out = update_multiphase(n, row[CPp[o]], row[CPm[o]], row[CP[o + kRowLen]],
                                         row[CP[o - kRowLen]], row[lzp], row[lzm], p == 1, p == v.X,
-                                        a.w);
+                                        w);
             }
             if (ok) P[o] = out;
         }
@@ -536,15 +550,17 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     auto store_plane = [&](int q) {
         if (q < x0 || q >= x1) return;
         const float* P = sF + (q % NP) * PS;
+        const bool own0 = zl >= z0 && zl < z1, own1 = zl + 1 >= z0 && zl + 1 < z1;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int r = wave + j * NW;
             const int y = ey0 + r;
             if (y < y0 || y >= y1) continue;
-            const int g = q * v.sX + y * v.sY + zl;
-            const int o = r * kRowLen + lane;
-            if (zl >= z0 && zl < z1) D[g] = P[o];
-            if (zl + 1 >= z0 && zl + 1 < z1) D[g + 1] = P[o + kRowPairs];
+            const rsrc_t rd = make_rsrc(D + (size_t)q * v.sX + (size_t)y * v.sY, (uint32_t)v.sY * 4u);
+            const int lo = r * kRowLen + lane;
+            // lanes outside the owned tile store to an out-of-range offset (dropped)
+            st_f32(P[lo], rd, own0 ? zl * 4 : -1);
+            st_f32(P[lo + kRowPairs], rd, own1 ? zl * 4 + 4 : -1);
         }
     };
 
@@ -587,6 +603,23 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
             __syncthreads();
         });
     }
+}
+
+
+template <int H, int KIND, int NW, int RPW>
+__global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
+    // XCD-aware tile numbering (grid is a multiple of 8)
+    const int per = gridDim.x >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= a.total) return;
+    int t = logical;
+    const int tz = t % a.nTz; t /= a.nTz;
+    const int ty = t % a.nTy; t /= a.nTy;
+    const int tx = t % a.nTx;
+    const int b = t / a.nTx;
+    const size_t off = (size_t)b * a.v.sB;
+    sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, a.codes + off, a.hm, a.ncls, a.v, a.w,
+                                 a.colour0, a.TX, tz, ty, tx);
 }
 
 template <int H, int NW, int RPW>
@@ -703,15 +736,13 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Sums of layer x of one padded volume V; returns them in thread 0.
 template <int KIND>
-__global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restrict__ F,
-                                                            const uint8_t* __restrict__ C,
-                                                            const float* __restrict__ hm, int ncls,
-                                                            Vol v, double* __restrict__ flux,
-                                                            double* __restrict__ conc) {
-    const int x = blockIdx.x, b = blockIdx.y;
-    const float* V = F + (size_t)b * v.sB;
-    const uint8_t* Cv = C + (size_t)b * v.sB;
+__device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
+                                                const uint8_t* __restrict__ Cv,
+                                                const float* __restrict__ hm, int ncls,
+                                                const Vol v, int x, double& sum_conc,
+                                                double& sum_flux) {
     const bool has_face = x < v.X - 1;
     double accC = 0.0, accF = 0.0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -747,8 +778,23 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
             sc += redC[i];
             sf += redF[i];
         }
+        sum_conc = sc;
+        sum_flux = sf;
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restrict__ F,
+                                                            const uint8_t* __restrict__ C,
+                                                            const float* __restrict__ hm, int ncls,
+                                                            Vol v, double* __restrict__ flux,
+                                                            double* __restrict__ conc) {
+    const int x = blockIdx.x, b = blockIdx.y;
+    double sc = 0.0, sf = 0.0;
+    layer_sum_block<KIND>(F + (size_t)b * v.sB, C + (size_t)b * v.sB, hm, ncls, v, x, sc, sf);
+    if (threadIdx.x == 0) {
         conc[(size_t)b * v.X + x] = sc;
-        if (has_face) flux[(size_t)b * (v.X - 1) + x] = sf;
+        if (x < v.X - 1) flux[(size_t)b * (v.X - 1) + x] = sf;
     }
 }
 
@@ -772,6 +818,241 @@ extern "C" int tf_layer_sums(const float* field, const uint8_t* codes, const flo
     else
         return fail(-1, "unknown kind %d", kind);
     TF_LAUNCH_CHECK("k_layer_sums");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// uneven batch: many two-phase volumes, each with its own convergence
+// ---------------------------------------------------------------------------
+// Volumes live back to back in two packed field buffers and one packed code
+// buffer.  A batch pass covers the tiles of every volume in one launch;
+// tiles of volumes that have stopped return at once.  Every active volume has
+// done the same number of iterations (they all start at 0 and advance
+// together), so one colour and one source buffer hold for the whole launch.
+// The per-volume convergence decision runs on the device (k_batch_check), so
+// a sequence of passes and checks needs no host round trip and can be
+// captured in a graph.
+struct BatchArgs {
+    float* fa;
+    float* fb;
+    const uint8_t* codes;
+    const tf_volume* vols;
+    const int* tile_vol;
+    const tf_volume_state* state;
+    int n_tiles;
+    int src_b;
+    int colour0;
+    int TX;
+};
+
+template <int H, int NW, int RPW>
+__global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
+    using Sh = SweepShape<H, NW, RPW>;
+    const int per = gridDim.x >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= a.n_tiles) return;
+    const int vi = a.tile_vol[logical];
+    if (!a.state[vi].active) return;
+    const tf_volume vol = a.vols[vi];
+    const Vol v = make_vol(vol.X, vol.Y, vol.Z);
+    const int nTz = (vol.Z + Sh::TZ - 1) / Sh::TZ;
+    const int nTy = (vol.Y + Sh::TY - 1) / Sh::TY;
+    int t = logical - vol.tile_first;
+    const int tz = t % nTz; t /= nTz;
+    const int ty = t % nTy;
+    const int tx = t / nTy;
+    const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
+    float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
+    sweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.offset, nullptr, 0, v, vol.omega,
+                                           a.colour0, a.TX, tz, ty, tx);
+}
+
+template <int H, int NW, int RPW>
+static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
+    using Sh = SweepShape<H, NW, RPW>;
+    if constexpr (Sh::TY < 1) {
+        return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
+    } else {
+        const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
+        if (lds > 160 * 1024)
+            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
+        const int grid = ((a.n_tiles + 7) / 8) * 8;
+        static bool attr_set = false;
+        if (!attr_set && lds > 64 * 1024) {
+            if (hipFuncSetAttribute((const void*)k_sweep_batch<H, NW, RPW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                (void)hipGetLastError();
+            attr_set = true;
+        }
+        hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        TF_LAUNCH_CHECK("k_sweep_batch");
+        return 0;
+    }
+}
+
+template <int H>
+static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
+    if (nw == 8 && rpw == 2) return launch_sweep_batch<H, 8, 2>(a, st);
+    if (nw == 8 && rpw == 3) return launch_sweep_batch<H, 8, 3>(a, st);
+    if (nw == 16 && rpw == 1) return launch_sweep_batch<H, 16, 1>(a, st);
+    if (nw == 4 && rpw == 4) return launch_sweep_batch<H, 4, 4>(a, st);
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled", nw, rpw);
+}
+
+static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {
+    default_tile(n_half, t);
+    if (tile) {
+        if (tile->waves > 0) t->waves = tile->waves;
+        if (tile->rows_per_wave > 0) t->rows_per_wave = tile->rows_per_wave;
+        if (tile->tx > 0) t->tx = tile->tx;
+    }
+    if (t->tx < 1 || t->tx > 65536) return fail(-1, "invalid tile tx=%d", t->tx);
+    if (t->waves * t->rows_per_wave <= 2 * n_half)
+        return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", t->waves,
+                    t->rows_per_wave, n_half);
+    return 0;
+}
+
+extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_tile* tile, int* count) {
+    if (!count) return fail(-1, "null pointer argument");
+    if (X < 1 || Y < 1 || Z < 1) return fail(-1, "invalid shape %dx%dx%d", X, Y, Z);
+    if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS) return fail(-1, "n_half=%d out of range", n_half);
+    tf_tile t;
+    if (int rc = resolve_tile(n_half, tile, &t)) return rc;
+    const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
+    const long long tz = kRowLen - 2 * n_half;
+    long long n = ((Z + tz - 1) / tz) * ((Y + ty - 1) / ty) * ((X + t.tx - 1) / t.tx);
+    if (n >= (1LL << 28)) return fail(-1, "too many tiles");
+    *count = (int)n;
+    return 0;
+}
+
+extern "C" int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* codes,
+                              const tf_volume* vols, const int* tile_vol, int n_tiles,
+                              const tf_volume_state* state, int src_buffer, int first_colour,
+                              int n_half, const tf_tile* tile, void* stream) {
+    if (!field_a || !field_b || !codes || !vols || !tile_vol || !state)
+        return fail(-1, "null pointer argument");
+    if (field_a == field_b) return fail(-1, "field_a and field_b must be distinct buffers");
+    if (n_tiles < 1 || n_tiles >= (1 << 28)) return fail(-1, "n_tiles=%d out of range", n_tiles);
+    if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
+    if (src_buffer != 0 && src_buffer != 1) return fail(-1, "src_buffer must be 0 or 1");
+    if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
+        return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
+    tf_tile t;
+    if (int rc = resolve_tile(n_half, tile, &t)) return rc;
+    BatchArgs a;
+    a.fa = field_a; a.fb = field_b; a.codes = codes; a.vols = vols; a.tile_vol = tile_vol;
+    a.state = state; a.n_tiles = n_tiles; a.src_b = src_buffer; a.colour0 = first_colour;
+    a.TX = t.tx;
+    hipStream_t st = (hipStream_t)stream;
+    switch (n_half) {
+        case 1: return dispatch_batch_shape<1>(a, t.waves, t.rows_per_wave, st);
+        case 2: return dispatch_batch_shape<2>(a, t.waves, t.rows_per_wave, st);
+        case 3: return dispatch_batch_shape<3>(a, t.waves, t.rows_per_wave, st);
+        case 4: return dispatch_batch_shape<4>(a, t.waves, t.rows_per_wave, st);
+        case 5: return dispatch_batch_shape<5>(a, t.waves, t.rows_per_wave, st);
+        default: return dispatch_batch_shape<6>(a, t.waves, t.rows_per_wave, st);
+    }
+}
+
+__global__ __launch_bounds__(kSumThreads) void k_batch_layer_sums(
+    const float* __restrict__ fa, const float* __restrict__ fb, const uint8_t* __restrict__ codes,
+    const tf_volume* __restrict__ vols, const int* __restrict__ layer_vol,
+    const tf_volume_state* __restrict__ state, int src_b, double* __restrict__ flux,
+    double* __restrict__ conc) {
+    const int l = blockIdx.x;
+    const int vi = layer_vol[l];
+    if (!state[vi].active) return;
+    const tf_volume vol = vols[vi];
+    const Vol v = make_vol(vol.X, vol.Y, vol.Z);
+    const int x = l - vol.sum_offset;
+    const float* F = (src_b ? fb : fa) + vol.offset;
+    double sc = 0.0, sf = 0.0;
+    layer_sum_block<TF_KIND_SOLVER>(F, codes + vol.offset, nullptr, 0, v, x, sc, sf);
+    if (threadIdx.x == 0) {
+        conc[l] = sc;
+        flux[l] = x < v.X - 1 ? sf : 0.0;
+    }
+}
+
+extern "C" int tf_batch_layer_sums(const float* field_a, const float* field_b, const uint8_t* codes,
+                                   const tf_volume* vols, const int* layer_vol, int n_layers,
+                                   const tf_volume_state* state, int src_buffer, double* flux_sum,
+                                   double* conc_sum, void* stream) {
+    if (!field_a || !field_b || !codes || !vols || !layer_vol || !state || !flux_sum || !conc_sum)
+        return fail(-1, "null pointer argument");
+    if (n_layers < 1 || n_layers > (1 << 30)) return fail(-1, "n_layers=%d out of range", n_layers);
+    if (src_buffer != 0 && src_buffer != 1) return fail(-1, "src_buffer must be 0 or 1");
+    hipLaunchKernelGGL(k_batch_layer_sums, dim3(n_layers), dim3(kSumThreads), 0, (hipStream_t)stream,
+                       field_a, field_b, codes, vols, layer_vol, state, src_buffer, flux_sum, conc_sum);
+    TF_LAUNCH_CHECK("k_batch_layer_sums");
+    return 0;
+}
+
+// Convergence decision of one volume, in float64, with the arithmetic of
+// ThroughTransportSolver.compute_metrics / SORSolver.check_convergence as
+// taufactor_amd evaluates them on the host (taufactor/taufactor.py:109-153,
+// 293-331): layer-mean fluxes, max/min/sequential mean, relative error,
+// D_rel, tau, the non-percolation rule, the NaN rule, the 2e-3 tau rule.
+__global__ void k_batch_check(const tf_volume* __restrict__ vols, int nvol,
+                              const double* __restrict__ flux, tf_volume_state* __restrict__ st,
+                              int src_b, int iters_per_check, int iter_limit, double conv_crit) {
+    const int vi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vi >= nvol) return;
+    tf_volume_state S = st[vi];
+    if (!S.active) return;
+    const tf_volume vol = vols[vi];
+    S.iters += iters_per_check;
+    const double n2 = (double)vol.Y * (double)vol.Z;
+    const double* F = flux + vol.sum_offset;
+    const int nf = vol.X - 1;
+    double fmax = -__builtin_inf(), fmin = __builtin_inf(), acc = 0.0;
+    for (int x = 0; x < nf; ++x) {
+        const double f = F[x] / n2;
+        fmax = f > fmax ? f : fmax;
+        fmin = f < fmin ? f : fmin;
+        acc += f;
+    }
+    const double mean = acc / (double)nf;
+    double rel = fmax != 0.0 ? (fmax - fmin) / fmax : __builtin_nan("");
+    double D_rel = mean * (double)vol.X / 1.0;
+    double tau = D_rel != 0.0 ? vol.D_mean / D_rel : __builtin_nan("");
+    if ((fmin == 0.0 || fmax == 0.0 || mean == 0.0) && !vol.spans) {
+        rel = 0.0;
+        D_rel = 0.0;
+        tau = 0.0;
+    }
+    if (mean != mean) rel = 0.0;
+    S.D_rel = D_rel;
+    S.rel_err = rel;
+    bool conv = false;
+    if (!(rel < conv_crit)) {
+        S.old_tau = tau;
+    } else if (!(__builtin_fabs(tau - S.old_tau) < 2e-3)) {
+        S.old_tau = tau;
+    } else {
+        conv = true;
+        if (tau == 0.0) tau = __builtin_inf();
+    }
+    S.tau = tau;
+    if (conv || S.iters >= iter_limit) {
+        S.active = 0;
+        S.converged = conv ? 1 : 0;
+        S.cur = src_b;
+    }
+    st[vi] = S;
+}
+
+extern "C" int tf_batch_check(const tf_volume* vols, int nvol, const double* flux_sum,
+                              tf_volume_state* state, int src_buffer, int iters_per_check,
+                              int iter_limit, double conv_crit, void* stream) {
+    if (!vols || !flux_sum || !state) return fail(-1, "null pointer argument");
+    if (nvol < 1) return fail(-1, "nvol=%d out of range", nvol);
+    if (src_buffer != 0 && src_buffer != 1) return fail(-1, "src_buffer must be 0 or 1");
+    hipLaunchKernelGGL(k_batch_check, dim3((nvol + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       vols, nvol, flux_sum, state, src_buffer, iters_per_check, iter_limit, conv_crit);
+    TF_LAUNCH_CHECK("k_batch_check");
     return 0;
 }
 

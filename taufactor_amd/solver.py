@@ -333,7 +333,8 @@ class ThroughTransportSolver(SORSolver):
         flux_1d = flux_sum / n2
         fl_max = np.max(flux_1d, axis=1)
         fl_min = np.min(flux_1d, axis=1)
-        mean_fl = np.mean(flux_1d, axis=1)
+        # sequential sum, the order the device-side batch check uses
+        mean_fl = np.cumsum(flux_1d, axis=1)[:, -1] / flux_1d.shape[1]
         relative_error = np.divide(fl_max - fl_min, fl_max,
                                    out=np.full_like(fl_max, np.nan), where=fl_max != 0)
 
