@@ -40,6 +40,10 @@ extern "C" {
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
+/* per-voxel diffusivity: `codes` is a float32 [B][X+2][Y+2][Z+2] array of
+ * voxel diffusivities (x ghosts copy the adjacent voxel, y/z ghosts 0);
+ * faces use the MultiPhase harmonic mean of the two voxels. */
+#define TF_KIND_FIELD 2
 
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by (128 - 2*n_half)
@@ -98,7 +102,7 @@ int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
  *   n_half   1..TF_MAX_FUSED_HALF_SWEEPS
  *   tile     NULL for the default tile shape                              */
 #define TF_MAX_FUSED_HALF_SWEEPS 6
-int tf_sweep(const float* src, float* dst, const uint8_t* codes,
+int tf_sweep(const float* src, float* dst, const void* codes,
              const float* hm_lut, int n_classes, int kind,
              int B, int X, int Y, int Z, float omega, int first_colour,
              int n_half, const tf_tile* tile, void* stream);
@@ -120,10 +124,38 @@ int tf_sweep_half_inplace(float* field, const uint8_t* codes,
  *            TF_KIND_SOLVER; hm*(c[x+1]-c[x]) for TF_KIND_MULTIPHASE)
  *   conc_sum device double [B][X]:   sum over (y,z) of c
  * The summation order is fixed, so repeated calls give identical bits.    */
-int tf_layer_sums(const float* field, const uint8_t* codes,
+int tf_layer_sums(const float* field, const void* codes,
                   const float* hm_lut, int n_classes, int kind,
                   int B, int X, int Y, int Z,
                   double* flux_sum, double* conc_sum, void* stream);
+
+/* Set-up of a per-voxel diffusivity volume (TF_KIND_FIELD).  No reference
+ * counterpart: it is MultiPhaseSolver (taufactor/taufactor.py:585-604) with a
+ * diffusivity map given per voxel instead of per label.
+ *   k     device float [B][X][Y][Z], voxel diffusivity >= 0 (0 = inert)
+ *   kpad  device float padded, written (x ghosts copy, y/z ghosts 0)
+ *   other arguments as tf_setup_solver; conductive means k > 0.          */
+int tf_setup_field(const float* k, const float* xinit, float* kpad,
+                   float* field_a, float* field_b, uint32_t* layer_count,
+                   int B, int X, int Y, int Z, void* stream);
+
+/* Gradient of D_rel with respect to every voxel diffusivity, from a
+ * converged field (no reference counterpart: the reference solves under
+ * torch.no_grad, taufactor/taufactor.py:172).  D_rel = (X/(Y*Z)) * W, W the
+ * dissipation sum_faces g (dc)^2 + sum_x-faces 2k (c - bc)^2, which the field
+ * minimises, so dD_rel/dk_v is the explicit derivative of W (one pass, no
+ * adjoint solve).  Inert voxels (k = 0) get the one-sided derivative.
+ *   grad  device float [B][X][Y][Z]                                        */
+int tf_grad_field(const float* field, const float* kpad, int B, int X, int Y, int Z,
+                  float* grad, void* stream);
+
+/* The same derivative summed per class of a labelled volume, layer by layer:
+ *   d_cls       device float [n_classes], diffusivity of each class (class 0 = 0)
+ *   grad_layers device double [B][X][n_classes]; summing over x gives
+ *               dD_rel/dD_class (fixed summation order)                    */
+int tf_grad_classes(const float* field, const uint8_t* codes, const float* d_cls,
+                    int n_classes, int B, int X, int Y, int Z, double* grad_layers,
+                    void* stream);
 
 /* ---- uneven batch (no reference counterpart: the reference batches only
  * volumes of one shape and stops when all have converged,

@@ -107,3 +107,90 @@ def multiphase_D_rel(img: np.ndarray, diffusivities: dict, **kw) -> float:
     for lbl, d in diffusivities.items():
         k[img == lbl] = float(np.float32(d))
     return solve_fixed_point(k, **kw)[2]
+
+
+def dissipation(k: np.ndarray, c: np.ndarray) -> float:
+    """W = sum_faces g (dc)^2 + sum_x-faces 2 k (c - bc)^2 of a field c."""
+    k = np.asarray(k, dtype=np.float64)
+    W = 0.0
+    for axis in range(3):
+        sl_a = [slice(None)] * 3
+        sl_b = [slice(None)] * 3
+        sl_a[axis] = slice(0, -1)
+        sl_b[axis] = slice(1, None)
+        g = _hm(k[tuple(sl_a)], k[tuple(sl_b)])
+        W += float(np.sum(g * (c[tuple(sl_b)] - c[tuple(sl_a)]) ** 2))
+    W += float(np.sum(2 * k[0] * (c[0] + 0.5) ** 2)) + float(np.sum(2 * k[-1] * (c[-1] - 0.5) ** 2))
+    return W
+
+
+def torch_D_rel(k):
+    """D_rel of a small voxel diffusivity field as a differentiable torch
+    float64 function of k (dense direct solve; all k > 0 assumed, so every
+    voxel is an unknown).  Its autograd gradient is the independent
+    reference of the adjoint kernel: it differentiates through the solve."""
+    import torch
+
+    k = k.to(torch.float64)
+    X, Y, Z = k.shape
+    n = X * Y * Z
+    idx = torch.arange(n).reshape(X, Y, Z)
+    A = torch.zeros((n, n), dtype=torch.float64)
+    b = torch.zeros(n, dtype=torch.float64)
+    for axis in range(3):
+        ia = idx.narrow(axis, 0, k.shape[axis] - 1).reshape(-1)
+        ib = idx.narrow(axis, 1, k.shape[axis] - 1).reshape(-1)
+        ka = k.narrow(axis, 0, k.shape[axis] - 1).reshape(-1)
+        kb = k.narrow(axis, 1, k.shape[axis] - 1).reshape(-1)
+        g = 2 * ka * kb / (ka + kb)
+        A = A.index_put((ia, ia), g, accumulate=True)
+        A = A.index_put((ib, ib), g, accumulate=True)
+        A = A.index_put((ia, ib), -g, accumulate=True)
+        A = A.index_put((ib, ia), -g, accumulate=True)
+    i0 = idx[0].reshape(-1)
+    i1 = idx[-1].reshape(-1)
+    g0 = 2 * k[0].reshape(-1)
+    g1 = 2 * k[-1].reshape(-1)
+    A = A.index_put((i0, i0), g0, accumulate=True)
+    A = A.index_put((i1, i1), g1, accumulate=True)
+    b = b.index_put((i0,), g0 * -0.5, accumulate=True)
+    b = b.index_put((i1,), g1 * 0.5, accumulate=True)
+    c = torch.linalg.solve(A, b)
+    # current through the x = 0 Dirichlet faces
+    Q = torch.sum(g0 * (c[i0] + 0.5))
+    return Q * X / (Y * Z)
+
+
+def energy_gradient(k: np.ndarray, c: np.ndarray) -> np.ndarray:
+    """dD_rel/dk from the dissipation identity (the formula the HIP kernel
+    evaluates), numpy float64, including the one-sided derivative at k = 0."""
+    k = np.asarray(k, dtype=np.float64)
+    X, Y, Z = k.shape
+    kp = np.pad(k, 1)
+    cp = np.pad(c.astype(np.float64), 1)
+    grad = np.zeros_like(k)
+    offs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for x in range(X):
+        for y in range(Y):
+            for z in range(Z):
+                kv, cv = k[x, y, z], c[x, y, z]
+                nb = []
+                for dx, dy, dz in offs:
+                    xx = x + dx
+                    if xx < 0 or xx >= X:
+                        continue
+                    ku = kp[xx + 1, y + dy + 1, z + dz + 1]
+                    if ku > 0:
+                        nb.append((ku, cp[xx + 1, y + dy + 1, z + dz + 1]))
+                bnd = ([-0.5] if x == 0 else []) + ([0.5] if x == X - 1 else [])
+                if kv > 0:
+                    g = sum((cu - cv) ** 2 * 2 * ku * ku / (kv + ku) ** 2 for ku, cu in nb)
+                    g += sum(2 * (cv - bc) ** 2 for bc in bnd)
+                else:
+                    vals = [cu for _, cu in nb] + bnd
+                    if not vals:
+                        continue
+                    cs = sum(vals) / len(vals)
+                    g = sum(2 * (v - cs) ** 2 for v in vals)
+                grad[x, y, z] = g
+    return grad * X / (Y * Z)

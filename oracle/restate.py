@@ -155,7 +155,7 @@ class OracleSolve:
         self.tau_x = np.divide(eps * dfc, flux_1d, out=np.full_like(dfc, np.nan), where=flux_1d != 0)
         for b in range(self.B):
             if fl_min[b] == 0 or fl_max[b] == 0 or mean_fl[b] == 0:
-                if not spans_x(np.isin(self.cpu_img[b], self.conductive_labels)):
+                if not _spans_any(self.cpu_img[b], self.conductive_labels):
                     rel[b] = 0
                     D_rel[b] = 0
                     tau[b] = 0
@@ -281,3 +281,44 @@ class OracleMultiPhaseSolver(OracleSolve):
     def vertical_flux(self):
         f = self.field
         return self.D_x[:, 1:-1] * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])
+
+
+class OracleFieldSolver(OracleMultiPhaseSolver):
+    """MultiPhaseSolver restated with the diffusivity map given per voxel
+    (taufactor/taufactor.py:585-620 with diff_map = k): the oracle of
+    taufactor_amd.FieldSolver.  D_mean is the mean voxel diffusivity."""
+
+    def __init__(self, k, omega=None, D_scaling=1, reduce="f64"):
+        k = expand_to_4d(np.asarray(k, dtype=np.float32))
+        self._k = k
+        self.Ds = {}
+        OracleSolve.__init__(self, k, omega, reduce)
+        self.D_0 = D_scaling
+        self.D_mean = k.reshape(k.shape[0], -1).astype(np.float64).mean(axis=1)
+
+    def mask(self, torch_img):
+        return (torch_img > 0).to(torch.float32)
+
+    def init_factor(self, torch_img, mask):
+        self.conductive_labels = None
+        dmap = pad(torch_img.clone(), [0, 0])
+        dmap[:, 0] = dmap[:, 1]
+        dmap[:, -1] = dmap[:, -2]
+        self.D_x = harmonic_mean(dmap[:, :-1, 1:-1, 1:-1], dmap[:, 1:, 1:-1, 1:-1])
+        self.D_y = harmonic_mean(dmap[:, 1:-1, :-1, 1:-1], dmap[:, 1:-1, 1:, 1:-1])
+        self.D_z = harmonic_mean(dmap[:, 1:-1, 1:-1, :-1], dmap[:, 1:-1, 1:-1, 1:])
+        factor = (self.D_x[:, :-1] + self.D_x[:, 1:] + self.D_y[:, :, :-1] + self.D_y[:, :, 1:] +
+                  self.D_z[:, :, :, :-1] + self.D_z[:, :, :, 1:])
+        factor[:, 0] += self.D_x[:, 0]
+        factor[:, -1] += self.D_x[:, -1]
+        factor[factor == 0] = torch.inf
+        return factor
+
+    def compute_metrics(self):
+        # the spanning test of a per-voxel map uses k > 0
+        self.conductive_labels = None
+        return super().compute_metrics()
+
+
+def _spans_any(img3d, labels):
+    return spans_x(img3d > 0) if labels is None else spans_x(np.isin(img3d, labels))

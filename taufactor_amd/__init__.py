@@ -6,7 +6,9 @@ include/tauhip.h) loaded through ctypes; PyTorch-ROCm holds the device
 buffers.  There is no CPU code path.
 """
 from .batch import BatchSolver
-from .solver import MultiPhaseSolver, SORSolver, Solver, ThroughTransportSolver
+from .autograd import effective_diffusivity
+from .solver import FieldSolver, MultiPhaseSolver, SORSolver, Solver, ThroughTransportSolver
 
-__all__ = ["Solver", "MultiPhaseSolver", "BatchSolver", "SORSolver", "ThroughTransportSolver"]
+__all__ = ["Solver", "MultiPhaseSolver", "BatchSolver", "FieldSolver", "effective_diffusivity",
+           "SORSolver", "ThroughTransportSolver"]
 __version__ = "0.1.0"

@@ -15,6 +15,7 @@ ABI_VERSION = 1
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
+KIND_FIELD = 2
 MAX_FUSED_HALF_SWEEPS = 6
 
 _c_int = ctypes.c_int
@@ -40,6 +41,9 @@ SIGNATURES = {
     "tf_layer_sums": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                _vp, _vp, _vp]),
     "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
+    "tf_setup_field": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_grad_field": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+    "tf_grad_classes": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "tf_sweep_tile_count": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(Tile),
                                      ctypes.POINTER(ctypes.c_int)]),
     "tf_batch_sweep": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _c_int, _c_int,

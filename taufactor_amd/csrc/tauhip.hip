@@ -383,7 +383,7 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
 struct SweepArgs {
     const float* src;
     float* dst;
-    const uint8_t* codes;
+    const void* codes;
     const float* hm;
     int ncls;
     Vol v;
@@ -415,22 +415,35 @@ struct SweepShape {
     static constexpr int PS = EY * kRowLen;  // floats (and code bytes) per plane
 };
 
+// per-voxel auxiliary data: a one-byte code (Solver, MultiPhase) or the
+// voxel diffusivity (per-voxel field)
+template <int KIND>
+using code_t = std::conditional_t<KIND == TF_KIND_FIELD, float, uint8_t>;
+
+// 2ab/(a+b), 0 when a+b == 0, with the reference's float32 operation order
+// ((2*a)*b)/(a+b) (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
+__device__ __forceinline__ float hmean(float a, float b) {
+    const float den = a + b;
+    return den > 0.0f ? ((2.0f * a) * b) / den : 0.0f;
+}
+
 template <int H, int KIND, int NW, int RPW>
 __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
-                                           const uint8_t* __restrict__ Cg,
+                                           const code_t<KIND>* __restrict__ Cg,
                                            const float* __restrict__ hm, int ncls, const Vol v,
                                            float w, int colour0, int TX, int tz, int ty, int tx) {
     using Sh = SweepShape<H, NW, RPW>;
+    using CT = code_t<KIND>;
     constexpr int EY = Sh::EY, TY = Sh::TY, TZ = Sh::TZ, NP = Sh::NP, PS = Sh::PS;
     static_assert(TY >= 1, "tile has no interior rows");
     extern __shared__ __align__(16) unsigned char smem[];
     float* sF = reinterpret_cast<float*>(smem);
-    uint8_t* sC = reinterpret_cast<uint8_t*>(sF + NP * PS);
+    CT* sC = reinterpret_cast<CT*>(sF + NP * PS);
     float* sT = reinterpret_cast<float*>(sC + NP * PS);
     (void)EY;
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
     const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
     const int ey0 = y0 - H, ez0 = z0 - H;
@@ -441,7 +454,7 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
 
     if (KIND == TF_KIND_SOLVER) {
         if (threadIdx.x < 16) sT[threadIdx.x] = threadIdx.x < 9 ? c_rcp[threadIdx.x] : 0.0f;
-    } else {
+    } else if (KIND == TF_KIND_MULTIPHASE) {
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
     }
 
@@ -449,36 +462,43 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     // Register slot of plane q is (q - S0) mod kPrefetch; every use below has
     // a compile-time slot, so the ring stays in registers.
     float pf[kPrefetch][RPW][2];
-    uint32_t pc[kPrefetch][RPW][2];
+    std::conditional_t<KIND == TF_KIND_FIELD, float, uint32_t> pc[kPrefetch][RPW][2];
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         // one buffer descriptor per padded row (the row is wave-uniform): z
-        // outside [0, Z+1] and rows outside [0, Y+1] read as 0 and cost no
-        // memory traffic
+        // outside [0, Z+1], rows outside [0, Y+1] and planes past the tile's
+        // last plane read as 0 and cost no memory traffic.  No branches: the
+        // waitcnt pass can then keep kPrefetch planes in flight.
+        const bool pok = p < lx1;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int y = ey0 + wave + j * NW;
-            const bool rok = y >= 0 && y <= v.Y + 1;
-            const size_t row = (size_t)p * v.sX + (size_t)(rok ? y : 0) * v.sY;
-            const rsrc_t rf = make_rsrc(S + row, rok ? (uint32_t)v.sY * 4u : 0u);
-            const rsrc_t rc = make_rsrc(Cg + row, rok ? (uint32_t)v.sY : 0u);
+            const bool ok = pok && y >= 0 && y <= v.Y + 1;
+            const size_t row = ok ? (size_t)p * v.sX + (size_t)y * v.sY : 0;
+            const rsrc_t rf = make_rsrc(S + row, ok ? (uint32_t)v.sY * 4u : 0u);
+            const rsrc_t rc = make_rsrc(Cg + row, ok ? (uint32_t)v.sY * (uint32_t)sizeof(CT) : 0u);
             pf[sl][j][0] = ld_f32(rf, zl * 4);
             pf[sl][j][1] = ld_f32(rf, zl * 4 + 4);
-            pc[sl][j][0] = ld_u8(rc, zl);
-            pc[sl][j][1] = ld_u8(rc, zl + 1);
+            if constexpr (KIND == TF_KIND_FIELD) {
+                pc[sl][j][0] = ld_f32(rc, zl * 4);
+                pc[sl][j][1] = ld_f32(rc, zl * 4 + 4);
+            } else {
+                pc[sl][j][0] = ld_u8(rc, zl);
+                pc[sl][j][1] = ld_u8(rc, zl + 1);
+            }
         }
     };
     auto commit_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         float* P = sF + (p % NP) * PS;
-        uint8_t* CP = sC + (p % NP) * PS;
+        CT* CP = sC + (p % NP) * PS;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int o = (wave + j * NW) * kRowLen + lane;
             P[o] = pf[sl][j][0];
             P[o + kRowPairs] = pf[sl][j][1];
-            CP[o] = (uint8_t)pc[sl][j][0];
-            CP[o + kRowPairs] = (uint8_t)pc[sl][j][1];
+            CP[o] = (CT)pc[sl][j][0];
+            CP[o + kRowPairs] = (CT)pc[sl][j][1];
         }
     };
 
@@ -492,9 +512,9 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
         float* P = sF + (p % NP) * PS;
         const float* Pm = sF + ((p - 1) % NP) * PS;
         const float* Pp = sF + ((p + 1) % NP) * PS;
-        const uint8_t* CP = sC + (p % NP) * PS;
-        const uint8_t* CPm = sC + ((p - 1) % NP) * PS;
-        const uint8_t* CPp = sC + ((p + 1) % NP) * PS;
+        const CT* CP = sC + (p % NP) * PS;
+        const CT* CPm = sC + ((p - 1) % NP) * PS;
+        const CT* CPp = sC + ((p + 1) % NP) * PS;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int r = wave + j * NW;
@@ -534,7 +554,7 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
                 s = s - n.c;
                 s = s * w;
                 out = n.c + s;
-            } else {
+            } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
                 const int l = CP[o];
                 const float* row = sT + l * ncls;
                 const int lzm = act ? CP[oo] : CP[on];
@@ -542,23 +562,32 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
                 out = update_multiphase(n, row[CPp[o]], row[CPm[o]], row[CP[o + kRowLen]],
                                         row[CP[o - kRowLen]], row[lzp], row[lzm], p == 1, p == v.X,
                                         w);
+            } else {
+                const float kc = CP[o];
+                const float kzm = act ? CP[oo] : CP[on];
+                const float kzp = act ? CP[on] : CP[oo];
+                out = update_multiphase(n, hmean(kc, CPp[o]), hmean(kc, CPm[o]),
+                                        hmean(kc, CP[o + kRowLen]), hmean(kc, CP[o - kRowLen]),
+                                        hmean(kc, kzp), hmean(kc, kzm), p == 1, p == v.X, w);
             }
             if (ok) P[o] = out;
         }
     };
 
     auto store_plane = [&](int q) {
-        if (q < x0 || q >= x1) return;
-        const float* P = sF + (q % NP) * PS;
+        // every lane stores; lanes outside the owned tile (plane, row or z)
+        // use an out-of-range offset, which drops the write
+        const bool qok = q >= x0 && q < x1;
+        const float* P = sF + (qok ? q % NP : 0) * PS;
         const bool own0 = zl >= z0 && zl < z1, own1 = zl + 1 >= z0 && zl + 1 < z1;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int r = wave + j * NW;
             const int y = ey0 + r;
-            if (y < y0 || y >= y1) continue;
-            const rsrc_t rd = make_rsrc(D + (size_t)q * v.sX + (size_t)y * v.sY, (uint32_t)v.sY * 4u);
+            const bool ok = qok && y >= y0 && y < y1;
+            const size_t row = ok ? (size_t)q * v.sX + (size_t)y * v.sY : 0;
+            const rsrc_t rd = make_rsrc(D + row, ok ? (uint32_t)v.sY * 4u : 0u);
             const int lo = r * kRowLen + lane;
-            // lanes outside the owned tile store to an out-of-range offset (dropped)
             st_f32(P[lo], rd, own0 ? zl * 4 : -1);
             st_f32(P[lo + kRowPairs], rd, own1 ? zl * 4 + 4 : -1);
         }
@@ -567,44 +596,39 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     const int S0 = max(x0 - (H - 1), 1);
     const int S1 = (x1 - 1) + (H - 1);
     using I0 = std::integral_constant<int, 0>;
-    // planes S0-1 .. S0+1 go straight to LDS; S0+2 .. S0+kPrefetch are put in flight
+    // planes S0-1 .. S0+1 go straight to LDS; S0+2 .. S0+kPrefetch+1 are put in flight
     issue_load(S0 - 1, I0{});
     commit_load(S0 - 1, I0{});
-    if (S0 < lx1) {
-        issue_load(S0, I0{});
-        commit_load(S0, I0{});
-    }
-    if (S0 + 1 < lx1) {
-        issue_load(S0 + 1, I0{});
-        commit_load(S0 + 1, I0{});
-    }
+    issue_load(S0, I0{});
+    commit_load(S0, I0{});
+    issue_load(S0 + 1, I0{});
+    commit_load(S0 + 1, I0{});
     static_for<2, kPrefetch + 2>([&](auto d_c) {
         constexpr int d = decltype(d_c)::value;
-        if (S0 + d < lx1) issue_load(S0 + d, std::integral_constant<int, d % kPrefetch>{});
+        issue_load(S0 + d, std::integral_constant<int, d % kPrefetch>{});
     });
     __syncthreads();
 
     // step s: put plane s+1+kPrefetch in flight, run half-sweep h on plane
-    // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS
+    // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS.
+    // Steps past S1 (to complete the last group of kPrefetch) do no work
+    // but keep the memory instructions unconditional.
     for (int s0 = S0; s0 <= S1; s0 += kPrefetch) {
         static_for<0, kPrefetch>([&](auto u_c) {
             constexpr int u = decltype(u_c)::value;
             const int s = s0 + u;
-            if (s > S1) return;
-            if (s + 1 + kPrefetch < lx1)
-                issue_load(s + 1 + kPrefetch, std::integral_constant<int, (u + 1) % kPrefetch>{});
+            issue_load(s + 1 + kPrefetch, std::integral_constant<int, (u + 1) % kPrefetch>{});
 #pragma unroll
             for (int h = 0; h < H; ++h) {
                 stage(h, s - h);
                 if (h + 1 < H) __syncthreads();
             }
             store_plane(s - (H - 1));
-            if (s + 2 < lx1) commit_load(s + 2, std::integral_constant<int, (u + 2) % kPrefetch>{});
+            commit_load(s + 2, std::integral_constant<int, (u + 2) % kPrefetch>{});
             __syncthreads();
         });
     }
 }
-
 
 template <int H, int KIND, int NW, int RPW>
 __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
@@ -618,14 +642,16 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const int tx = t % a.nTx;
     const int b = t / a.nTx;
     const size_t off = (size_t)b * a.v.sB;
-    sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, a.codes + off, a.hm, a.ncls, a.v, a.w,
-                                 a.colour0, a.TX, tz, ty, tx);
+    sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
+                                 static_cast<const code_t<KIND>*>(a.codes) + off, a.hm, a.ncls, a.v,
+                                 a.w, a.colour0, a.TX, tz, ty, tx);
 }
 
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
     using Sh = SweepShape<H, NW, RPW>;
-    size_t bytes = (size_t)Sh::NP * Sh::PS * (sizeof(float) + 1);
+    const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
+    size_t bytes = (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
     bytes += kind == TF_KIND_MULTIPHASE ? (size_t)ncls * ncls * sizeof(float) : 16 * sizeof(float);
     return bytes;
 }
@@ -691,13 +717,14 @@ static void default_tile(int H, tf_tile* t) {
     t->tx = 64;
 }
 
-extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, const float* hm_lut,
+extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const float* hm_lut,
                         int n_classes, int kind, int B, int X, int Y, int Z, float omega,
                         int first_colour, int n_half, const tf_tile* tile, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
     if (!src || !dst || !codes) return fail(-1, "null pointer argument");
     if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
-    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE) return fail(-1, "unknown kind %d", kind);
+    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD)
+        return fail(-1, "unknown kind %d", kind);
     if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 64))
         return fail(-1, "multiphase sweep needs hm_lut and 1..64 classes (got %d)", n_classes);
     if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
@@ -720,7 +747,9 @@ extern "C" int tf_sweep(const float* src, float* dst, const uint8_t* codes, cons
     hipStream_t st = (hipStream_t)stream;
     if (kind == TF_KIND_SOLVER)
         return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, B, t.waves, t.rows_per_wave, st);
-    return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
+    if (kind == TF_KIND_MULTIPHASE)
+        return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
+    return dispatch_sweep<TF_KIND_FIELD>(n_half, a, B, t.waves, t.rows_per_wave, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -739,7 +768,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Sums of layer x of one padded volume V; returns them in thread 0.
 template <int KIND>
 __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
-                                                const uint8_t* __restrict__ Cv,
+                                                const code_t<KIND>* __restrict__ Cv,
                                                 const float* __restrict__ hm, int ncls,
                                                 const Vol v, int x, double& sum_conc,
                                                 double& sum_flux) {
@@ -755,10 +784,12 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
             if (has_face) {
                 const float cn = V[o + v.sX];
                 float d = cn - c;
-                if (KIND == TF_KIND_SOLVER) {
+                if constexpr (KIND == TF_KIND_SOLVER) {
                     if (Cv[o] == 0 || Cv[o + v.sX] == 0) d = 0.0f;
-                } else {
+                } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
                     d = hm[Cv[o] * ncls + Cv[o + v.sX]] * d;
+                } else {
+                    d = hmean(Cv[o], Cv[o + v.sX]) * d;
                 }
                 accF += (double)d;
             }
@@ -785,7 +816,7 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
 
 template <int KIND>
 __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restrict__ F,
-                                                            const uint8_t* __restrict__ C,
+                                                            const code_t<KIND>* __restrict__ C,
                                                             const float* __restrict__ hm, int ncls,
                                                             Vol v, double* __restrict__ flux,
                                                             double* __restrict__ conc) {
@@ -798,7 +829,7 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
     }
 }
 
-extern "C" int tf_layer_sums(const float* field, const uint8_t* codes, const float* hm_lut,
+extern "C" int tf_layer_sums(const float* field, const void* codes, const float* hm_lut,
                              int n_classes, int kind, int B, int X, int Y, int Z, double* flux_sum,
                              double* conc_sum, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
@@ -811,13 +842,231 @@ extern "C" int tf_layer_sums(const float* field, const uint8_t* codes, const flo
     dim3 grid(X, B);
     if (kind == TF_KIND_SOLVER)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_SOLVER>, grid, dim3(kSumThreads), 0, st, field,
-                           codes, hm_lut, n_classes, v, flux_sum, conc_sum);
+                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
     else if (kind == TF_KIND_MULTIPHASE)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_MULTIPHASE>, grid, dim3(kSumThreads), 0, st, field,
-                           codes, hm_lut, n_classes, v, flux_sum, conc_sum);
+                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
+    else if (kind == TF_KIND_FIELD)
+        hipLaunchKernelGGL(k_layer_sums<TF_KIND_FIELD>, grid, dim3(kSumThreads), 0, st, field,
+                           static_cast<const float*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
     else
         return fail(-1, "unknown kind %d", kind);
     TF_LAUNCH_CHECK("k_layer_sums");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// per-voxel diffusivity field: set-up
+// ---------------------------------------------------------------------------
+// kpad[b][x][y][z] = k of the voxel, x ghosts copy the adjacent voxel's k,
+// y/z ghosts 0 (the reference's diff_map padding, taufactor/taufactor.py:590-592);
+// both field buffers get the initial field, conductive = k > 0.
+__global__ void k_setup_field(const float* __restrict__ k, const float* __restrict__ xinit,
+                              float* __restrict__ kpad, float* __restrict__ fa,
+                              float* __restrict__ fb, Vol v, int B) {
+    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long total = (long long)B * v.sB;
+    if (gid >= total) return;
+    int b = (int)(gid / v.sB);
+    int r = (int)(gid - (long long)b * v.sB);
+    int xp = r / v.sX;
+    int r2 = r - xp * v.sX;
+    int yp = r2 / v.sY;
+    int zp = r2 - yp * v.sY;
+    const float* K = k + (size_t)b * v.X * v.Y * v.Z;
+    const bool yz_in = (yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z);
+    float f = 0.0f, kk = 0.0f;
+    if (yz_in && xp >= 1 && xp <= v.X) {
+        kk = K[((size_t)(xp - 1) * v.Y + (yp - 1)) * v.Z + (zp - 1)];
+        f = (kk > 0.0f ? 1.0f : 0.0f) * xinit[xp - 1];
+    } else if (yz_in) {
+        const int x = xp == 0 ? 0 : v.X - 1;
+        kk = K[((size_t)x * v.Y + (yp - 1)) * v.Z + (zp - 1)];
+        f = xp == 0 ? -1.0f : 1.0f;
+    }
+    const size_t o = (size_t)b * v.sB + r;
+    kpad[o] = kk;
+    fa[o] = f;
+    fb[o] = f;
+}
+
+__global__ void k_layer_count_field(const float* __restrict__ k, uint32_t* __restrict__ cnt, Vol v) {
+    int x = blockIdx.x, b = blockIdx.y;
+    const float* K = k + ((size_t)b * v.X + x) * v.Y * v.Z;
+    int n = v.Y * v.Z;
+    uint32_t acc = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += K[i] > 0.0f;
+    __shared__ uint32_t red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = red[0];
+}
+
+extern "C" int tf_setup_field(const float* k, const float* xinit, float* kpad, float* field_a,
+                              float* field_b, uint32_t* layer_count, int B, int X, int Y, int Z,
+                              void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!k || !xinit || !kpad || !field_a || !field_b || !layer_count)
+        return fail(-1, "null pointer argument");
+    if (field_a == field_b) return fail(-1, "field_a and field_b must be distinct buffers");
+    if (X > 65535 || B > 65535) return fail(-1, "grid too large");
+    Vol v = make_vol(X, Y, Z);
+    hipStream_t st = (hipStream_t)stream;
+    long long total = (long long)B * v.sB;
+    hipLaunchKernelGGL(k_setup_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, k, xinit,
+                       kpad, field_a, field_b, v, B);
+    TF_LAUNCH_CHECK("k_setup_field");
+    hipLaunchKernelGGL(k_layer_count_field, dim3(X, B), dim3(256), 0, st, k, layer_count, v);
+    TF_LAUNCH_CHECK("k_layer_count_field");
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// implicit-adjoint gradient of D_rel with respect to voxel diffusivities
+// ---------------------------------------------------------------------------
+// At the fixed point c of the discrete operator (faces g = hm(k_a, k_b),
+// Dirichlet half-faces 2*k_v to bc = -+0.5), D_rel = (X/(Y Z)) * W with
+// W = sum_faces g (dc)^2 + sum_x-faces 2 k_v (c_v - bc)^2 the dissipation at
+// unit potential difference; c minimises W, so dW/dk_v is the explicit
+// derivative (no adjoint solve needed):
+//   k_v > 0: sum over internal faces (c_u - c_v)^2 * 2 k_u^2/(k_v+k_u)^2
+//            + sum over its x faces 2 (c_v - bc)^2
+//   k_v = 0: one-sided derivative; v joins at c* = the mean of its
+//            conductive neighbours and Dirichlet faces (weights all 2):
+//            sum 2 (c_u - c*)^2 + sum 2 (c* - bc)^2
+// Evaluated in float64 from the float32 field.
+template <typename KFUN>
+__device__ __forceinline__ double voxel_grad(const float* __restrict__ V, int o, const Vol v, int x,
+                                             KFUN kat) {
+    const int offs[6] = {v.sX, -v.sX, v.sY, -v.sY, 1, -1};
+    const double kv = kat(o);
+    const double cv = V[o];
+    double g = 0.0;
+    if (kv > 0.0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;  // Dirichlet faces below
+            const double ku = kat(o + offs[i]);
+            if (ku <= 0.0) continue;
+            const double d = (double)V[o + offs[i]] - cv;
+            const double s = kv + ku;
+            g += d * d * (2.0 * ku * ku) / (s * s);
+        }
+        if (x == 0) g += 2.0 * (cv + 0.5) * (cv + 0.5);
+        if (x == v.X - 1) g += 2.0 * (cv - 0.5) * (cv - 0.5);
+    } else {
+        double sum = 0.0;
+        int n = 0;
+        double cu[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            cu[i] = 0.0;
+            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;
+            if (kat(o + offs[i]) > 0.0) {
+                cu[i] = V[o + offs[i]];
+                sum += cu[i];
+                ++n;
+            }
+        }
+        if (x == 0) { sum += -0.5; ++n; }
+        if (x == v.X - 1) { sum += 0.5; ++n; }
+        if (n == 0) return 0.0;
+        const double cs = sum / n;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;
+            if (kat(o + offs[i]) > 0.0) g += 2.0 * (cu[i] - cs) * (cu[i] - cs);
+        }
+        if (x == 0) g += 2.0 * (cs + 0.5) * (cs + 0.5);
+        if (x == v.X - 1) g += 2.0 * (cs - 0.5) * (cs - 0.5);
+    }
+    return g;
+}
+
+__global__ void k_grad_field(const float* __restrict__ F, const float* __restrict__ K, Vol v, int B,
+                             double scale, float* __restrict__ grad) {
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int bx = blockIdx.z;
+    const int b = bx / v.X, x = bx - b * v.X;
+    if (z >= v.Z) return;
+    const float* V = F + (size_t)b * v.sB;
+    const float* Kb = K + (size_t)b * v.sB;
+    const int o = (x + 1) * v.sX + (y + 1) * v.sY + (z + 1);
+    const double g = voxel_grad(V, o, v, x, [&](int i) { return (double)Kb[i]; });
+    grad[(((size_t)b * v.X + x) * v.Y + y) * v.Z + z] = (float)(g * scale);
+}
+
+extern "C" int tf_grad_field(const float* field, const float* kpad, int B, int X, int Y, int Z,
+                             float* grad, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!field || !kpad || !grad) return fail(-1, "null pointer argument");
+    if ((long long)B * X > 65535 || Y > 65535) return fail(-1, "grid too large for gradient");
+    Vol v = make_vol(X, Y, Z);
+    const double scale = (double)X / ((double)Y * (double)Z);
+    hipLaunchKernelGGL(k_grad_field, dim3((Z + 255) / 256, Y, B * X), dim3(256), 0,
+                       (hipStream_t)stream, field, kpad, v, B, scale, grad);
+    TF_LAUNCH_CHECK("k_grad_field");
+    return 0;
+}
+
+// per-class sums of the voxel gradient for one x layer; one block per (x, b)
+constexpr int kMaxGradClasses = 64;
+__global__ __launch_bounds__(256) void k_grad_classes(const float* __restrict__ F,
+                                                      const uint8_t* __restrict__ C,
+                                                      const float* __restrict__ dcls, int ncls,
+                                                      Vol v, double scale,
+                                                      double* __restrict__ out) {
+    const int x = blockIdx.x, b = blockIdx.y;
+    const float* V = F + (size_t)b * v.sB;
+    const uint8_t* Cv = C + (size_t)b * v.sB;
+    __shared__ double part[256 / 64][kMaxGradClasses];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < (256 / 64) * kMaxGradClasses; i += 256) (&part[0][0])[i] = 0.0;
+    __syncthreads();
+    auto kat = [&](int i) { return (double)dcls[Cv[i]]; };
+    for (int y = 0; y < v.Y; ++y) {
+        for (int z0 = 0; z0 < v.Z; z0 += 256) {
+            const int z = z0 + threadIdx.x;
+            double g = 0.0;
+            int cl = 0;
+            if (z < v.Z) {
+                const int o = (x + 1) * v.sX + (y + 1) * v.sY + (z + 1);
+                cl = Cv[o];
+                g = voxel_grad(V, o, v, x, kat);
+            }
+            // classes are few: reduce each class present in the wave in turn
+            for (int c = 0; c < ncls; ++c) {
+                double gc = cl == c ? g : 0.0;
+                gc = wave_sum(gc);
+                if (lane == 0) part[wave][c] += gc;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < ncls) {
+        double s2 = 0.0;
+        for (int w = 0; w < 256 / 64; ++w) s2 += part[w][threadIdx.x];
+        out[((size_t)b * v.X + x) * ncls + threadIdx.x] = s2 * scale;
+    }
+}
+
+extern "C" int tf_grad_classes(const float* field, const uint8_t* codes, const float* d_cls,
+                               int n_classes, int B, int X, int Y, int Z, double* grad_layers,
+                               void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!field || !codes || !d_cls || !grad_layers) return fail(-1, "null pointer argument");
+    if (n_classes < 1 || n_classes > kMaxGradClasses) return fail(-1, "n_classes=%d out of range", n_classes);
+    if (X > 65535 || B > 65535) return fail(-1, "grid too large for gradient");
+    Vol v = make_vol(X, Y, Z);
+    const double scale = (double)X / ((double)Y * (double)Z);
+    hipLaunchKernelGGL(k_grad_classes, dim3(X, B), dim3(256), 0, (hipStream_t)stream, field, codes,
+                       d_cls, n_classes, v, scale, grad_layers);
+    TF_LAUNCH_CHECK("k_grad_classes");
     return 0;
 }
 

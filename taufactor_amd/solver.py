@@ -31,7 +31,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "MultiPhaseSolver"]
+__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "MultiPhaseSolver", "FieldSolver"]
 
 # half-sweeps fused per HBM pass unless a solver is told otherwise
 DEFAULT_HALF_SWEEPS_PER_PASS = 4
@@ -520,12 +520,13 @@ class MultiPhaseSolver(ThroughTransportSolver):
         self._labels, self._label_counts = labels, counts
         # diffusivity per label as the reference's float32 diff_map holds it
         d_label = np.array([np.float32(self.Ds.get(lbl.item(), 0.0)) for lbl in labels], dtype=np.float32)
-        # classes: 0 = zero diffusivity (and ghosts), then one per distinct value
-        pos = sorted(set(float(d) for d in d_label if d > 0))
-        if len(pos) + 1 > self.MAX_CLASSES:
-            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct non-zero diffusivities are supported")
-        d_cls = np.array([0.0] + pos, dtype=np.float32)
-        cls_of_label = np.array([0 if d == 0 else 1 + pos.index(float(d)) for d in d_label], dtype=np.uint8)
+        # classes: 0 = the zero-diffusivity ghost class, then one per label
+        # present in the image (labels stay separate so that gradients can be
+        # reported per label)
+        if len(labels) + 1 > self.MAX_CLASSES:
+            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct labels are supported")
+        d_cls = np.concatenate([np.zeros(1, dtype=np.float32), d_label]).astype(np.float32)
+        cls_of_label = np.arange(1, len(labels) + 1, dtype=np.uint8)
         # face conductance table: hm(a,b) = 2*a*b/(a+b), 0 where a+b == 0
         # (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
         a = d_cls[:, None]
@@ -534,6 +535,7 @@ class MultiPhaseSolver(ThroughTransportSolver):
         with np.errstate(divide="ignore", invalid="ignore"):
             hm = np.where(denom > 0, (2 * a * b) / denom, np.float32(0)).astype(np.float32)
         self._d_cls = d_cls
+        self._d_cls_dev = torch.from_numpy(d_cls).to(self.device)
         self._n_classes = len(d_cls)
         cls_img = self._class_image(labels, cls_of_label)
         self._input_dev = torch.from_numpy(cls_img).to(self.device)
@@ -570,6 +572,21 @@ class MultiPhaseSolver(ThroughTransportSolver):
             return torch.zeros_like(img)
         conductive = torch.tensor(self.conductive_labels, dtype=self.precision, device=self.device)
         return torch.isin(img, conductive).to(self.precision)
+
+    def grad_D_eff(self) -> dict:
+        """dD_eff/dD_label for every label of the image, float64 [B] each, from
+        the current (converged) field.  No reference counterpart (the
+        reference solves under torch.no_grad).  The derivative of the discrete
+        fixed point: see include/tauhip.h, tf_grad_classes."""
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        out = torch.empty((B, X, self._n_classes), dtype=torch.float64, device=self.device)
+        with torch.cuda.device(self.device):
+            st = self._lib.tf_grad_classes(_lib.ptr(self.field), _lib.ptr(self._codes),
+                                           _lib.ptr(self._d_cls_dev), self._n_classes, B, X, Y, Z,
+                                           _lib.ptr(out), self._stream())
+            _lib.check(st, "tf_grad_classes")
+            per_cls = out.cpu().numpy().sum(axis=1) * self.D_0
+        return {int(lbl): per_cls[:, i + 1] for i, lbl in enumerate(self._labels)}
 
     # ---- reference attributes, derived on demand from the class codes ----
     def _face_maps(self):
@@ -611,3 +628,90 @@ class MultiPhaseSolver(ThroughTransportSolver):
         """taufactor/taufactor.py:615-620"""
         f = self.field
         return self.D_x[:, 1:-1] * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])
+
+
+class FieldSolver(ThroughTransportSolver):
+    """Through-transport solve with one diffusivity per voxel.
+
+    No reference counterpart as a class: it is the reference's
+    ``MultiPhaseSolver`` (taufactor/taufactor.py:508-620) with the label ->
+    diffusivity map replaced by a diffusivity per voxel, i.e. the same
+    harmonic-mean face conductances, Dirichlet doubling, check and metrics;
+    ``D_mean`` is the mean voxel diffusivity.  ``grad_D_eff()`` returns
+    dD_eff/dk for every voxel (implicit adjoint, one pass).
+
+    Args:
+        k: voxel diffusivities >= 0 ([X,Y], [X,Y,Z] or [B,X,Y,Z]); a numpy
+           array or a torch tensor (a float32 tensor on the solver's device is
+           used in place).
+        D_scaling: reference diffusivity D_0 (D_eff = D_0 * D_rel).
+    """
+
+    kind = _lib.KIND_FIELD
+
+    def __init__(self, k, D_scaling=1, omega=None, device="cuda"):
+        self._k_tensor = k if isinstance(k, torch.Tensor) else None
+        k_np = k.detach().cpu().numpy() if isinstance(k, torch.Tensor) else k
+        if not isinstance(k_np, np.ndarray):
+            raise TypeError("Error: input image must be a NumPy array!")
+        if k_np.size and (not np.all(np.isfinite(k_np)) or k_np.min() < 0):
+            raise ValueError("voxel diffusivities must be finite and >= 0")
+        self.conductive_labels = None
+        super().__init__(k_np, omega=omega, device=device)
+        self.D_0 = D_scaling
+        self.D_mean = self.cpu_img.reshape(self.batch_size, -1).astype(np.float32).astype(np.float64).mean(axis=1)
+
+    def _spans(self, b):
+        if b not in self._spans_cache:
+            mask = np.ascontiguousarray(self.cpu_img[b] > 0, dtype=np.uint8)
+            X, Y, Z = mask.shape
+            r = self._lib.tf_spans_x(mask.ctypes.data, X, Y, Z)
+            if r < 0:
+                _lib.check(r, "tf_spans_x")
+            self._spans_cache[b] = bool(r)
+        return self._spans_cache[b]
+
+    def _setup_device(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        t = self._k_tensor
+        if t is not None and t.device == self.device and t.dtype == torch.float32:
+            self._input_dev = t.detach().reshape(B, X, Y, Z).contiguous()
+        else:
+            self._input_dev = torch.from_numpy(
+                np.ascontiguousarray(self.cpu_img, dtype=np.float32)).to(self.device)
+        shape = (B, X + 2, Y + 2, Z + 2)
+        self._codes = torch.empty(shape, dtype=torch.float32, device=self.device)
+        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
+        self._xinit_dev = self._xinit()
+
+    def _run_setup(self):
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        st = self._lib.tf_setup_field(_lib.ptr(self._input_dev), _lib.ptr(self._xinit_dev),
+                                      _lib.ptr(self._codes), _lib.ptr(self._fields[0]),
+                                      _lib.ptr(self._fields[1]), _lib.ptr(self._count),
+                                      B, X, Y, Z, self._stream())
+        _lib.check(st, "tf_setup_field")
+        self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+
+    def grad_D_eff(self) -> torch.Tensor:
+        """dD_eff/dk for every voxel, float32 [B, X, Y, Z] on the device, from
+        the current (converged) field."""
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        g = torch.empty((B, X, Y, Z), dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            st = self._lib.tf_grad_field(_lib.ptr(self.field), _lib.ptr(self._codes), B, X, Y, Z,
+                                         _lib.ptr(g), self._stream())
+            _lib.check(st, "tf_grad_field")
+        if self.D_0 != 1:
+            g.mul_(self.D_0)
+        return g
+
+    def vertical_flux(self):
+        k = self._codes
+        kx = k[:, 1:-2, 1:-1, 1:-1]
+        kn = k[:, 2:-1, 1:-1, 1:-1]
+        den = kx + kn
+        hm = torch.where(den > 0, (2 * kx * kn) / den, torch.zeros_like(den))
+        f = self.field
+        return hm * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])
