@@ -18,7 +18,18 @@ cpu_baseline  the CPU oracle (an op-for-op torch restatement of the
             reference loop) on this host, on a bounded sample: a fixed number
             of iterations of the same 512^3 volume, with one convergence check
 
+Other workloads (not part of the default line):
+  --config batch256       256 binary cubes of edge U[32,160] (smoothed noise,
+                          porosity U[0.3,0.7], sigma U[1,5]): one BatchSolver
+                          vs 256 sequential Solver solves; solves/s and the
+                          largest relative D_eff difference
+  --config multiphase320  320^3 three-label volume {0:0, 1:1, 2:0.1}: forward
+                          solve and per-phase dD_eff/dD (one gradient pass)
+  --config field128x64    64 volumes of 128^3 with k = sigmoid(4 z): batched
+                          forward solve and per-voxel dD_eff/dk
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                       [--config solver512|batch256|multiphase320|field128x64]
 """
 from __future__ import annotations
 
@@ -76,6 +87,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tile", type=str, default=None, help="ty,tz,tx")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config", default="solver512",
+                    choices=["solver512", "batch256", "multiphase320", "field128x64"])
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -88,6 +101,13 @@ def main(argv=None) -> int:
 
     import taufactor_amd as tau
     from taufactor_amd import workloads
+
+    if args.config == "batch256":
+        return bench_batch(args)
+    if args.config == "multiphase320":
+        return bench_multiphase(args)
+    if args.config == "field128x64":
+        return bench_field(args)
 
     t0 = time.perf_counter()
     img = workloads.smoothed_binary(args.size, 0.35, 4.0, 0)
@@ -178,6 +198,150 @@ def main(argv=None) -> int:
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_iters)
         log(f"cpu baseline {out['cpu_baseline']['value']} GVoxel-updates/s")
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def _line(metric, value, unit, steps, warmup, ms, config, higher=True, dtype="f32", **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic (seeded, generated on the host)",
+           "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_batch(args) -> int:
+    """256 uneven volumes: one BatchSolver against 256 sequential Solver solves."""
+    import numpy as np
+    import torch
+
+    import taufactor_amd as tau
+    from taufactor_amd import workloads
+
+    t0 = time.perf_counter()
+    vols = [v for v, _, _ in workloads.uneven_batch(256, seed=0)]
+    nvox = sum(v.size for v in vols)
+    log(f"{len(vols)} volumes, {nvox / 1e6:.1f} M voxels ({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda", 0)
+    b = tau.BatchSolver(vols, device=dev)
+    b.solve()  # warm-up (graph capture, autotuned launches)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    b.reset()
+    b.solve()
+    torch.cuda.synchronize(dev)
+    t_batch = time.perf_counter() - t0
+    log(f"batch: {t_batch:.3f} s, iterations {int(b.iters.min())}..{int(b.iters.max())}")
+    # sequential plain solves, volumes already resident on the device
+    solvers = [tau.Solver(v, device=dev) for v in vols]
+    for s in solvers[:4]:
+        s.solve(verbose=False)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    d_seq, it_seq = [], []
+    for s in solvers:
+        s.reset()
+        s.solve(verbose=False)
+        d_seq.append(float(s.D_eff[0]))
+        it_seq.append(s.iter)
+    torch.cuda.synchronize(dev)
+    t_seq = time.perf_counter() - t0
+    d_seq = np.array(d_seq)
+    rel = np.abs(b.D_eff.astype(np.float64) - d_seq) / np.maximum(np.abs(d_seq), 1e-30)
+    same_iters = bool(np.array_equal(np.array(it_seq), b.iters))
+    log(f"sequential: {t_seq:.3f} s; batch speed-up {t_seq / t_batch:.2f}x; max rel D_eff diff {rel.max():.2e}")
+    vox_updates = sum(v.size * it / 2 for v, it in zip(vols, b.iters))
+    _line("solves/s batched vs plain @ D_eff rel err<=1e-3", round(len(vols) / t_batch, 3), "solves/s",
+          1, 1, round(t_batch * 1e3, 3),
+          {"workload": "256 binary cubes, edge U[32,160], smoothed noise sigma U[1,5], porosity U[0.3,0.7], seed 0",
+           "voxels": nvox},
+          batch={"solves_per_s": round(len(vols) / t_batch, 3), "time_s": round(t_batch, 4),
+                 "GVoxel_updates_per_s": round(vox_updates / t_batch / 1e9, 3)},
+          sequential={"solves_per_s": round(len(vols) / t_seq, 3), "time_s": round(t_seq, 4)},
+          speedup=round(t_seq / t_batch, 3), max_rel_D_eff_diff=float(rel.max()),
+          same_stopping_iterations=same_iters)
+    return 0
+
+
+def bench_multiphase(args) -> int:
+    """320^3 three-label volume: forward solve and per-phase gradient."""
+    import torch
+
+    import taufactor_amd as tau
+    from taufactor_amd import workloads
+
+    t0 = time.perf_counter()
+    img = workloads.three_phase(320, seed=0)
+    log(f"volume {img.shape} fractions {[(img == l).mean().round(3) for l in range(3)]} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda", 0)
+    Ds = {0: 0.0, 1: 1.0, 2: 0.1}
+    s = tau.MultiPhaseSolver(img, dict(Ds), device=dev)
+    s.solve(verbose=False)
+    s._launch_events = []
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    s.reset()
+    s.solve(verbose=False)
+    torch.cuda.synchronize(dev)
+    t_fwd = time.perf_counter() - t0
+    ev, s._launch_events = s._launch_events, None
+    t0 = time.perf_counter()
+    g = s.grad_D_eff()
+    torch.cuda.synchronize(dev)
+    t_bwd = time.perf_counter() - t0
+    nvox = img.size
+    sweep_ms = sum(a.elapsed_time(b) for a, b, _ in ev) / sum(n for _, _, n in ev)
+    gvus = nvox * s.iter / 2 / t_fwd / 1e9
+    log(f"forward {t_fwd:.3f} s ({s.iter} iterations, D_eff {float(s.D_eff[0]):.6f}); "
+        f"backward {t_bwd * 1e3:.2f} ms; dD_eff/dD {g[1][0]:.5f}, {g[2][0]:.5f}")
+    _line("GVoxel-updates/s (MultiPhaseSolver forward) + backward/forward time", round(gvus, 3),
+          "GVoxel-updates/s", 1, 1, round(t_fwd * 1e3, 3),
+          {"workload": "MultiPhaseSolver 320^3, labels 0/1/2 at 0.4/0.3/0.3 from smoothed noise sigma 3, "
+                       "D {0:0, 1:1.0, 2:0.1}, conv_crit 1e-2", "iterations": s.iter},
+          D_eff=float(s.D_eff[0]), tau=float(s.tau[0]),
+          grad_D_eff={"1": float(g[1][0]), "2": float(g[2][0])},
+          backward_ms=round(t_bwd * 1e3, 3), backward_over_forward=round(t_bwd / t_fwd, 5),
+          sweep_kernel_mean_ms=round(sweep_ms, 4),
+          sweep_kernel_GBps_alg=round(9 * nvox / (sweep_ms / 1e3) / 1e9, 1))
+    return 0
+
+
+def bench_field(args) -> int:
+    """64 volumes of 128^3 with per-voxel k: batched forward and per-voxel gradient."""
+    import torch
+
+    import taufactor_amd as tau
+    from taufactor_amd import workloads
+
+    t0 = time.perf_counter()
+    k = workloads.sigmoid_k(64, 128, seed=0)
+    log(f"k field {k.shape} ({time.perf_counter() - t0:.1f} s)")
+    dev = torch.device("cuda", 0)
+    kt = torch.from_numpy(k).to(dev)
+    s = tau.FieldSolver(kt, device=dev)
+    s.solve(verbose=False)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    s.reset()
+    s.solve(verbose=False)
+    torch.cuda.synchronize(dev)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = s.grad_D_eff()
+    torch.cuda.synchronize(dev)
+    t_bwd = time.perf_counter() - t0
+    nvox = k.size
+    gvus = nvox * s.iter / 2 / t_fwd / 1e9
+    log(f"forward {t_fwd:.3f} s ({s.iter} iterations); backward {t_bwd * 1e3:.2f} ms")
+    _line("GVoxel-updates/s (per-voxel k forward, 64 volumes) + backward/forward time", round(gvus, 3),
+          "GVoxel-updates/s", 1, 1, round(t_fwd * 1e3, 3),
+          {"workload": "64 volumes of 128^3, k = sigmoid(4 z), z standardised smoothed noise sigma 3, "
+                       "conv_crit 1e-2 (all volumes stop together, reference batch semantics)",
+           "iterations": s.iter},
+          D_eff_mean=float(s.D_eff.mean()), backward_ms=round(t_bwd * 1e3, 3),
+          backward_over_forward=round(t_bwd / t_fwd, 5),
+          grad_abs_mean=float(g.abs().mean()))
     return 0
 
 

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 1
+#define TF_ABI_VERSION 2
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -49,7 +49,7 @@ extern "C" {
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by (128 - 2*n_half)
  * voxels in z along tx planes of x; one wavefront per row.  Compiled shapes
  * (waves x rows_per_wave): 8x2, 8x3, 16x1, 4x4.  Zero fields select the
- * defaults (8x2 for n_half <= 3, 8x3 above; tx 64). */
+ * defaults (8x2 for n_half <= 3 and for TF_KIND_FIELD, 8x3 above; tx 64). */
 typedef struct tf_tile {
     int waves, rows_per_wave, tx;
 } tf_tile;
@@ -61,6 +61,9 @@ int tf_abi_version(void);
 /* Thread-local message for the last non-zero status. */
 const char* tf_last_error(void);
 
+/* set-up flags: y and z periodic (PeriodicSolver / PeriodicMultiPhaseSolver) */
+#define TF_SETUP_PERIODIC_YZ 1
+
 /* Set-up of a two-phase volume.  Replaces SORSolver.__init__ field/prefactor
  * construction: ThroughTransportSolver.init_field (taufactor/taufactor.py:282-291),
  * SORSolver._pad (:227-238), Solver.init_conductive_neighbours (:402-410)
@@ -71,21 +74,35 @@ const char* tf_last_error(void);
  *   codes   device uint8 padded, written (ghosts 0)
  *   field_a, field_b  device float padded, both written with the initial field
  *   layer_count device uint32 [B][X], written with the number of
- *           conductive voxels per x layer.                               */
+ *           conductive voxels per x layer.
+ *   flags   TF_SETUP_PERIODIC_YZ: neighbour counts wrap round in y and z
+ *           (PeriodicSolver.init_conductive_neighbours,
+ *           taufactor/taufactor.py:493-499)                               */
 int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t* codes,
                     float* field_a, float* field_b, uint32_t* layer_count,
-                    int B, int X, int Y, int Z, void* stream);
+                    int B, int X, int Y, int Z, int flags, void* stream);
 
 /* Set-up of a labelled volume for the face-conductance stencil.  Replaces
  * MultiPhaseSolver.return_mask / init_conductive_neighbours
  * (taufactor/taufactor.py:571-604) and the shared field set-up above.
  *   cls      device uint8 [B][X][Y][Z], class index per voxel (< n_classes)
  *   cls_conductive device uint8 [n_classes], 1 if the class diffusivity > 0
+ *   flags    TF_SETUP_PERIODIC_YZ: y/z ghost codes copy the opposite
+ *            interior layer (PeriodicMultiPhaseSolver.init_conductive_neighbours,
+ *            taufactor/taufactor.py:626-650)
  *   Other arguments as tf_setup_solver.                                    */
 int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
                         int n_classes, const float* xinit, uint8_t* codes,
                         float* field_a, float* field_b, uint32_t* layer_count,
-                        int B, int X, int Y, int Z, void* stream);
+                        int B, int X, int Y, int Z, int flags, void* stream);
+
+/* Periodic boundaries: every y/z ghost of every x plane takes the value of
+ * the opposite interior layer (corners both wraps).  Replaces
+ * apply_boundary_conditions of PeriodicSolver / PeriodicMultiPhaseSolver
+ * (taufactor/taufactor.py:501-505, 652-656), called before every
+ * iteration: a periodic solve alternates tf_wrap_ghosts on the source buffer
+ * and a one-half-sweep tf_sweep.                                          */
+int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream);
 
 /* n_half over-relaxed red-black half-sweeps (reference iterations), the
  * first one on colour first_colour, fused in one HBM pass: reads src,

@@ -322,3 +322,57 @@ class OracleFieldSolver(OracleMultiPhaseSolver):
 
 def _spans_any(img3d, labels):
     return spans_x(img3d > 0) if labels is None else spans_x(np.isin(img3d, labels))
+
+
+def _wrap_yz(field: torch.Tensor) -> None:
+    """y/z ghosts := opposite interior layer (taufactor/taufactor.py:501-505)."""
+    field[:, :, 0, :] = field[:, :, -2, :]
+    field[:, :, -1, :] = field[:, :, 1, :]
+    field[:, :, :, 0] = field[:, :, :, -2]
+    field[:, :, :, -1] = field[:, :, :, 1]
+
+
+class OraclePeriodicSolver(OracleSolver):
+    """taufactor/taufactor.py:481-505: neighbour counts wrap round in y and z,
+    y/z ghosts refreshed before every iteration."""
+
+    def init_factor(self, torch_img, mask):
+        p = pad(mask, [2, 2])[:, :, 1:-1, 1:-1]  # x ghosts only
+        nn = torch.zeros_like(p)
+        for dim in range(1, 4):
+            for dr in (1, -1):
+                nn += torch.roll(p, dr, dim)
+        nn = nn[:, 1:-1]
+        nn[mask == 0] = torch.inf
+        nn[nn == 0] = torch.inf
+        return nn
+
+    def step(self):
+        _wrap_yz(self.field)
+        super().step()
+
+
+class OraclePeriodicMultiPhaseSolver(OracleMultiPhaseSolver):
+    """taufactor/taufactor.py:623-656"""
+
+    def init_factor(self, torch_img, mask):
+        dmap = torch.zeros_like(torch_img)
+        for phase, d in self.Ds.items():
+            dmap[torch_img == phase] = d
+        dmap = pad(dmap, [0, 0])
+        dmap[:, 0] = dmap[:, 1]
+        dmap[:, -1] = dmap[:, -2]
+        _wrap_yz(dmap)
+        self.D_x = harmonic_mean(dmap[:, :-1, 1:-1, 1:-1], dmap[:, 1:, 1:-1, 1:-1])
+        self.D_y = harmonic_mean(dmap[:, 1:-1, :-1, 1:-1], dmap[:, 1:-1, 1:, 1:-1])
+        self.D_z = harmonic_mean(dmap[:, 1:-1, 1:-1, :-1], dmap[:, 1:-1, 1:-1, 1:])
+        factor = (self.D_x[:, :-1] + self.D_x[:, 1:] + self.D_y[:, :, :-1] + self.D_y[:, :, 1:] +
+                  self.D_z[:, :, :, :-1] + self.D_z[:, :, :, 1:])
+        factor[:, 0] += self.D_x[:, 0]
+        factor[:, -1] += self.D_x[:, -1]
+        factor[factor == 0] = torch.inf
+        return factor
+
+    def step(self):
+        _wrap_yz(self.field)
+        super().step()

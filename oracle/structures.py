@@ -70,3 +70,8 @@ def staircase(n: int) -> np.ndarray:
     for i in range(n):
         img[i, i:i + 2, i:i + 2] = 1
     return img
+
+
+def diagonals_2d_features(n: int, features: int) -> np.ndarray:
+    """create_2d_diagonals(N, features) (taufactor/utils.py:196-203)."""
+    return diagonals_2d(n, features)

@@ -11,11 +11,12 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
 KIND_FIELD = 2
+SETUP_PERIODIC_YZ = 1
 MAX_FUSED_HALF_SWEEPS = 6
 
 _c_int = ctypes.c_int
@@ -31,9 +32,11 @@ class Tile(ctypes.Structure):
 SIGNATURES = {
     "tf_abi_version": (_c_int, []),
     "tf_last_error": (ctypes.c_char_p, []),
-    "tf_setup_solver": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_setup_solver": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                 _vp]),
     "tf_setup_multiphase": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp,
-                                     _c_int, _c_int, _c_int, _c_int, _vp]),
+                                     _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_wrap_ghosts": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                           _c_float, _c_int, _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_sweep_half_inplace": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,

@@ -119,7 +119,7 @@ class BatchSolver:
             st = self._lib.tf_setup_solver(
                 _lib.ptr(self._mask_dev) + self._mask_offsets[v], _lib.ptr(self._xinit_dev) + 4 * s0,
                 _lib.ptr(self._codes) + o, _lib.ptr(self._fa) + 4 * o, _lib.ptr(self._fb) + 4 * o,
-                _lib.ptr(self._count) + 4 * s0, 1, X, Y, Z, self._stream())
+                _lib.ptr(self._count) + 4 * s0, 1, X, Y, Z, 0, self._stream())
             _lib.check(st, "tf_setup_solver")
 
     def reset(self):

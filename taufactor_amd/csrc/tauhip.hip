@@ -130,7 +130,8 @@ __constant__ float c_rcp[9] = {0.0f, 1.0f, 0.5f, 1.0f / 3.0f, 0.25f,
 template <int MODE>
 __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
                         const float* __restrict__ xinit, uint8_t* __restrict__ codes,
-                        float* __restrict__ fa, float* __restrict__ fb, Vol v, int B) {
+                        float* __restrict__ fa, float* __restrict__ fb, Vol v, int B,
+                        int periodic) {
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long total = (long long)B * v.sB;
     if (gid >= total) return;
@@ -156,14 +157,22 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
         f = (cond ? 1.0f : 0.0f) * xinit[x];
         if (MODE == 0) {
             if (cond) {
-                // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410)
+                // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410),
+                // or y/z wrapped round (PeriodicSolver, taufactor/taufactor.py:493-499)
                 int nn = 0;
                 nn += (x + 1 < v.X) ? (m_at(x + 1, y, z) != 0) : 2;
                 nn += (x >= 1) ? (m_at(x - 1, y, z) != 0) : 2;
-                nn += (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
-                nn += (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
-                nn += (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
-                nn += (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
+                if (periodic) {
+                    nn += m_at(x, (y + 1) % v.Y, z) != 0;
+                    nn += m_at(x, (y + v.Y - 1) % v.Y, z) != 0;
+                    nn += m_at(x, y, (z + 1) % v.Z) != 0;
+                    nn += m_at(x, y, (z + v.Z - 1) % v.Z) != 0;
+                } else {
+                    nn += (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
+                    nn += (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
+                    nn += (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
+                    nn += (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
+                }
                 code = (uint8_t)nn;  // 0 stays 0 = inf
             }
         } else {
@@ -179,6 +188,14 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
             // diff_map[:,0] = diff_map[:,1] (taufactor/taufactor.py:591-592)
             int x = xp == 0 ? 0 : v.X - 1;
             code = (uint8_t)m_at(x, yp - 1, zp - 1);
+        }
+        if (MODE == 1 && periodic) {
+            // y/z ghosts copy the opposite interior layer, after the x copy
+            // (PeriodicMultiPhaseSolver, taufactor/taufactor.py:633-638)
+            const int y = (yp == 0 ? v.Y : (yp == v.Y + 1 ? 1 : yp)) - 1;
+            const int z = (zp == 0 ? v.Z : (zp == v.Z + 1 ? 1 : zp)) - 1;
+            const int x = xp == 0 ? 0 : (xp == v.X + 1 ? v.X - 1 : xp - 1);
+            code = (uint8_t)m_at(x, y, z);
         }
     }
     size_t o = (size_t)b * v.sB + r;
@@ -212,8 +229,9 @@ __global__ void k_layer_count(const uint8_t* __restrict__ mask, const uint8_t* _
 template <int MODE>
 static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const float* xinit,
                         uint8_t* codes, float* fa, float* fb, uint32_t* cnt,
-                        int B, int X, int Y, int Z, void* stream) {
+                        int B, int X, int Y, int Z, int flags, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (flags & ~TF_SETUP_PERIODIC_YZ) return fail(-1, "unknown set-up flags 0x%x", flags);
     if (!mask || !xinit || !codes || !fa || !fb || !cnt || (MODE == 1 && !cls_cond))
         return fail(-1, "null pointer argument");
     if (fa == fb) return fail(-1, "field_a and field_b must be distinct buffers");
@@ -223,7 +241,8 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
     int threads = 256;
     long long blocks = (total + threads - 1) / threads;
     hipLaunchKernelGGL(k_setup<MODE>, dim3((unsigned)blocks), dim3(threads), 0, st,
-                       mask, cls_cond, xinit, codes, fa, fb, v, B);
+                       mask, cls_cond, xinit, codes, fa, fb, v, B,
+                       (flags & TF_SETUP_PERIODIC_YZ) ? 1 : 0);
     TF_LAUNCH_CHECK("k_setup");
     hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, B), dim3(256), 0, st, mask, cls_cond, cnt, v);
     TF_LAUNCH_CHECK("k_layer_count");
@@ -232,18 +251,55 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
 
 extern "C" int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t* codes,
                                float* field_a, float* field_b, uint32_t* layer_count,
-                               int B, int X, int Y, int Z, void* stream) {
+                               int B, int X, int Y, int Z, int flags, void* stream) {
     return setup_common<0>(mask, nullptr, xinit, codes, field_a, field_b, layer_count,
-                           B, X, Y, Z, stream);
+                           B, X, Y, Z, flags, stream);
+}
+
+// y/z ghosts of every x plane := the opposite interior layer (ghost corners
+// take both wraps), as SORSolver.apply_boundary_conditions of the periodic
+// solvers does before each iteration (taufactor/taufactor.py:501-505, 652-656)
+__global__ void k_wrap_ghosts(float* __restrict__ F, Vol v, int B) {
+    // one thread per ghost position of one (b, x) plane: the border of the
+    // (Y+2) x (Z+2) plane, 2(Y+2) + 2Z positions
+    const int per_plane = 2 * (v.Y + 2) + 2 * v.Z;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= per_plane) return;
+    const int bx = blockIdx.y;
+    int yp, zp;
+    if (i < 2 * (v.Y + 2)) {
+        yp = i >> 1;
+        zp = (i & 1) ? v.Z + 1 : 0;
+    } else {
+        const int j = i - 2 * (v.Y + 2);
+        zp = 1 + (j >> 1);
+        yp = (j & 1) ? v.Y + 1 : 0;
+    }
+    const int ys = yp == 0 ? v.Y : (yp == v.Y + 1 ? 1 : yp);
+    const int zs = zp == 0 ? v.Z : (zp == v.Z + 1 ? 1 : zp);
+    float* P = F + (size_t)bx * v.sX;
+    P[yp * v.sY + zp] = P[ys * v.sY + zs];
+}
+
+extern "C" int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!field) return fail(-1, "null pointer argument");
+    if ((long long)B * (X + 2) > 65535) return fail(-1, "grid too large for ghost wrap");
+    Vol v = make_vol(X, Y, Z);
+    const int per_plane = 2 * (Y + 2) + 2 * Z;
+    hipLaunchKernelGGL(k_wrap_ghosts, dim3((per_plane + 255) / 256, B * (X + 2)), dim3(256), 0,
+                       (hipStream_t)stream, field, v, B);
+    TF_LAUNCH_CHECK("k_wrap_ghosts");
+    return 0;
 }
 
 extern "C" int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
                                    int n_classes, const float* xinit, uint8_t* codes,
                                    float* field_a, float* field_b, uint32_t* layer_count,
-                                   int B, int X, int Y, int Z, void* stream) {
+                                   int B, int X, int Y, int Z, int flags, void* stream) {
     if (n_classes < 1 || n_classes > 256) return fail(-1, "n_classes=%d out of range", n_classes);
     return setup_common<1>(cls, cls_conductive, xinit, codes, field_a, field_b, layer_count,
-                           B, X, Y, Z, stream);
+                           B, X, Y, Z, flags, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -711,9 +767,11 @@ static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream
     }
 }
 
-static void default_tile(int H, tf_tile* t) {
+static void default_tile(int H, int kind, tf_tile* t) {
     t->waves = 8;
-    t->rows_per_wave = H <= 3 ? 2 : 3;
+    // 24 rows when deeper blocking needs the taller halo, unless the 4-byte
+    // per-voxel diffusivity of TF_KIND_FIELD leaves too little LDS for them
+    t->rows_per_wave = (H <= 3 || kind == TF_KIND_FIELD) ? 2 : 3;
     t->tx = 64;
 }
 
@@ -731,7 +789,7 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
         return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
     tf_tile t;
-    default_tile(n_half, &t);
+    default_tile(n_half, kind, &t);
     if (tile) {
         if (tile->waves > 0) t.waves = tile->waves;
         if (tile->rows_per_wave > 0) t.rows_per_wave = tile->rows_per_wave;
@@ -1149,7 +1207,7 @@ static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
 }
 
 static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {
-    default_tile(n_half, t);
+    default_tile(n_half, TF_KIND_SOLVER, t);
     if (tile) {
         if (tile->waves > 0) t->waves = tile->waves;
         if (tile->rows_per_wave > 0) t->rows_per_wave = tile->rows_per_wave;

@@ -31,7 +31,8 @@ import torch
 
 from . import _lib
 
-__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "MultiPhaseSolver", "FieldSolver"]
+__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "MultiPhaseSolver",
+           "PeriodicMultiPhaseSolver", "FieldSolver"]
 
 # half-sweeps fused per HBM pass unless a solver is told otherwise
 DEFAULT_HALF_SWEEPS_PER_PASS = 4
@@ -51,6 +52,7 @@ class SORSolver(ABC):
     """
 
     kind: int = _lib.KIND_SOLVER
+    _setup_flags: int = 0
 
     def __init__(self, img: np.ndarray, omega: float | None = None, precision=None, device="cuda"):
         self.cpu_img = self._expand_to_4d(img)
@@ -444,7 +446,7 @@ class Solver(ThroughTransportSolver):
         st = self._lib.tf_setup_solver(_lib.ptr(self._input_dev), _lib.ptr(self._xinit_dev),
                                        _lib.ptr(self._codes), _lib.ptr(self._fields[0]),
                                        _lib.ptr(self._fields[1]), _lib.ptr(self._count),
-                                       B, X, Y, Z, self._stream())
+                                       B, X, Y, Z, self._setup_flags, self._stream())
         _lib.check(st, "tf_setup_solver")
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
@@ -553,7 +555,7 @@ class MultiPhaseSolver(ThroughTransportSolver):
                                            self._n_classes, _lib.ptr(self._xinit_dev),
                                            _lib.ptr(self._codes), _lib.ptr(self._fields[0]),
                                            _lib.ptr(self._fields[1]), _lib.ptr(self._count),
-                                           B, X, Y, Z, self._stream())
+                                           B, X, Y, Z, self._setup_flags, self._stream())
         _lib.check(st, "tf_setup_multiphase")
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
@@ -715,3 +717,48 @@ class FieldSolver(ThroughTransportSolver):
         hm = torch.where(den > 0, (2 * kx * kn) / den, torch.zeros_like(den))
         f = self.field
         return hm * (f[:, 2:-1, 1:-1, 1:-1] - f[:, 1:-2, 1:-1, 1:-1])
+
+
+class _PeriodicYZ:
+    """Periodic y/z boundaries: the y/z ghosts are refreshed from the opposite
+    interior layer before every iteration (apply_boundary_conditions,
+    taufactor/taufactor.py:501-505, 652-656), so the solve alternates one
+    ghost wrap and one half-sweep."""
+
+    _setup_flags = _lib.SETUP_PERIODIC_YZ
+
+    def apply_boundary_conditions(self):
+        st = self._lib.tf_wrap_ghosts(_lib.ptr(self.field), self.batch_size, self.Nx, self.Ny, self.Nz,
+                                      self._stream())
+        _lib.check(st, "tf_wrap_ghosts")
+
+    def _advance(self, n_half: int) -> None:
+        ev = self._launch_events
+        if ev is not None:
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+        saved, self._launch_events = self._launch_events, None
+        per = self.half_sweeps_per_pass
+        self.half_sweeps_per_pass = 1
+        try:
+            it = self.iter
+            for i in range(n_half):
+                self.apply_boundary_conditions()
+                self.iter = it + i  # colour of this iteration
+                SORSolver._advance(self, 1)
+            self.iter = it
+        finally:
+            self.half_sweeps_per_pass = per
+            self._launch_events = saved
+        if ev is not None:
+            t1.record()
+            ev.append((t0, t1, n_half))
+
+
+class PeriodicSolver(_PeriodicYZ, Solver):
+    """Two-phase solver with periodic y/z boundaries (taufactor/taufactor.py:481-505)."""
+
+
+class PeriodicMultiPhaseSolver(_PeriodicYZ, MultiPhaseSolver):
+    """Multi-phase solver with periodic y/z boundaries (taufactor/taufactor.py:623-656)."""

@@ -100,4 +100,4 @@ def test_autograd_function_returns_kernel_gradient():
     s.solve(verbose=False, conv_crit=1e-3)
     g = s.grad_D_eff()
     assert torch.equal(gk[0], g[0]) and torch.allclose(gk[1], 2 * g[1])
-    np.testing.assert_allclose(D.cpu().numpy(), s.D_rel, rtol=0, atol=0)
+    np.testing.assert_allclose(D.detach().cpu().numpy(), s.D_rel, rtol=0, atol=0)

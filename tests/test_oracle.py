@@ -187,3 +187,37 @@ def test_spans_x_matches_bruteforce():
         shape = tuple(rng.integers(1, 7, size=3))
         m = rng.random(shape) < rng.uniform(0.2, 0.7)
         assert spans_x(m) == _spans_bruteforce(m)
+
+
+# ---- periodic solvers (tests/test_taufactor.py:80-106, 226-247 and the notebook table) ----
+def test_periodic_known_answers():
+    from oracle.restate import OraclePeriodicMultiPhaseSolver, OraclePeriodicSolver
+
+    N = 20
+    img = np.ones((N, N, N))
+    img[:, 0] = 0
+    assert np.around(OraclePeriodicSolver(img).solve().tau, 5) == 1.0
+    img = np.zeros((N, N, N))
+    img[:2] = 1
+    assert OraclePeriodicSolver(img).solve(iter_limit=1000).tau == np.inf
+    img = np.zeros((N, N, N))
+    img[:, 0:10, 0:10] = 1
+    assert np.around(OraclePeriodicSolver(img).solve().tau, 5) == 1
+    assert np.around(OraclePeriodicMultiPhaseSolver(np.ones((N, N, N)), {1: 1.0}).solve(iter_limit=1000).tau, 4) == 1.0
+    st = staircase(20)
+    a = OraclePeriodicSolver(st).solve(iter_limit=1000)
+    b = OraclePeriodicMultiPhaseSolver(st, {0: 0.0, 1: 1.0}).solve(iter_limit=1000)
+    assert np.isclose(float(a.tau[0]), float(b.tau[0]), atol=1e-3)
+
+
+@pytest.mark.parametrize("which", ["PeriodicSolver", "PeriodicMultiPhaseSolver"])
+def test_periodic_notebook_table(goldens, which):
+    from oracle.restate import OraclePeriodicMultiPhaseSolver, OraclePeriodicSolver
+    from oracle.structures import diagonals_2d_features
+
+    row = next(r for r in goldens["periodic_diagonal2d64_conv1e-3"]["rows"] if r["solver"] == which)
+    img = diagonals_2d_features(64, 2)
+    cls = OraclePeriodicSolver if which == "PeriodicSolver" else OraclePeriodicMultiPhaseSolver
+    s = cls(img, reduce="f32").solve(conv_crit=1e-3)
+    assert s.iter == row["iterations"]
+    assert abs(float(s.tau[0]) - row["tau"]) <= 1.5e-6 * row["tau"]
