@@ -68,6 +68,8 @@ class BatchSolver:
         soff = 0
         mask_offsets = []
         for v, (X, Y, Z) in enumerate(self.shapes):
+            if (X + 2) * (Y + 2) * (Z + 2) * 4 >= 2 ** 31:
+                raise ValueError(f"volume {v} of shape {(X, Y, Z)} is too large for a batch (2 GiB per volume)")
             w = 2 - math.pi / (1.5 * X) if omega is None else omega
             vols[v]["offset"] = off
             vols[v]["X"], vols[v]["Y"], vols[v]["Z"] = X, Y, Z

@@ -87,14 +87,16 @@ constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer descriptor 
 __device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferFlags);
 }
-__device__ __forceinline__ float ld_f32(rsrc_t r, int byte_off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+// byte_off: per-lane offset (range-checked against the record count);
+// soff: wave-uniform offset added to the address
+__device__ __forceinline__ float ld_f32(rsrc_t r, int byte_off, int soff = 0) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, soff, 0));
 }
-__device__ __forceinline__ uint32_t ld_u8(rsrc_t r, int byte_off) {
-    return __builtin_amdgcn_raw_buffer_load_b8(r, byte_off, 0, 0);
+__device__ __forceinline__ uint32_t ld_u8(rsrc_t r, int byte_off, int soff = 0) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, byte_off, soff, 0);
 }
-__device__ __forceinline__ void st_f32(float val, rsrc_t r, int byte_off) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), r, byte_off, 0, 0);
+__device__ __forceinline__ void st_f32(float val, rsrc_t r, int byte_off, int soff = 0) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), r, byte_off, soff, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -504,8 +506,6 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
     const int ey0 = y0 - H, ez0 = z0 - H;
     const int zl = ez0 + 2 * lane;  // z of this lane's even element
-    const bool zok0 = zl >= 0 && zl <= v.Z + 1;
-    const bool zok1 = zl + 1 >= 0 && zl + 1 <= v.Z + 1;
     const int lx1 = min(x1 + H, v.X + 2);
 
     if (KIND == TF_KIND_SOLVER) {
@@ -517,37 +517,59 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     // ---- plane loads: global -> registers (kPrefetch planes in flight) -> LDS ----
     // Register slot of plane q is (q - S0) mod kPrefetch; every use below has
     // a compile-time slot, so the ring stays in registers.
+    //
+    // Global memory is addressed through one buffer descriptor per tile row,
+    // built once: base = that row in plane 0, record count = the rest of the
+    // volume (0 for rows outside [0, Y+1], and for stores outside the owned
+    // rows).  The lane offset is plane * plane_bytes + z * elem_bytes.  Lanes
+    // whose z is outside [0, Z+1], planes past the tile's last plane and
+    // stores outside the owned tile use an out-of-range lane offset: loads
+    // read 0, stores are dropped, no memory traffic.  No branches around
+    // memory instructions, so the waitcnt pass can keep kPrefetch planes in
+    // flight.
+    constexpr int kOff = 0x7fffff00;  // lane offset past any record count
+    const int plane_f = v.sX * 4, plane_c = v.sX * (int)sizeof(CT);
+    const int vol_f = (int)(v.sB * 4), vol_c = (int)(v.sB * (long long)sizeof(CT));
+    rsrc_t rf[RPW], rc[RPW], rd[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j) {
+        const int y = ey0 + wave + j * NW;
+        const bool rok = y >= 0 && y <= v.Y + 1;
+        const bool own = y >= y0 && y < y1;
+        const int yr = rok ? y : 0;
+        rf[j] = make_rsrc(S + (size_t)yr * v.sY, rok ? (uint32_t)(vol_f - yr * v.sY * 4) : 0u);
+        rc[j] = make_rsrc(Cg + (size_t)yr * v.sY,
+                          rok ? (uint32_t)(vol_c - yr * v.sY * (int)sizeof(CT)) : 0u);
+        rd[j] = make_rsrc(D + (size_t)yr * v.sY, own ? (uint32_t)(vol_f - yr * v.sY * 4) : 0u);
+    }
+    const bool zok0 = zl >= 0 && zl <= v.Z + 1, zok1 = zl + 1 >= 0 && zl + 1 <= v.Z + 1;
+    const bool own0 = zl >= z0 && zl < z1, own1 = zl + 1 >= z0 && zl + 1 < z1;
+
     float pf[kPrefetch][RPW][2];
     std::conditional_t<KIND == TF_KIND_FIELD, float, uint32_t> pc[kPrefetch][RPW][2];
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
-        // one buffer descriptor per padded row (the row is wave-uniform): z
-        // outside [0, Z+1], rows outside [0, Y+1] and planes past the tile's
-        // last plane read as 0 and cost no memory traffic.  No branches: the
-        // waitcnt pass can then keep kPrefetch planes in flight.
         const bool pok = p < lx1;
+        const int bf = p * plane_f + zl * 4, bc = p * plane_c + zl * (int)sizeof(CT);
+        const int f0 = (pok && zok0) ? bf : kOff, f1 = (pok && zok1) ? bf + 4 : kOff;
+        const int c0 = (pok && zok0) ? bc : kOff, c1 = (pok && zok1) ? bc + (int)sizeof(CT) : kOff;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const int y = ey0 + wave + j * NW;
-            const bool ok = pok && y >= 0 && y <= v.Y + 1;
-            const size_t row = ok ? (size_t)p * v.sX + (size_t)y * v.sY : 0;
-            const rsrc_t rf = make_rsrc(S + row, ok ? (uint32_t)v.sY * 4u : 0u);
-            const rsrc_t rc = make_rsrc(Cg + row, ok ? (uint32_t)v.sY * (uint32_t)sizeof(CT) : 0u);
-            pf[sl][j][0] = ld_f32(rf, zl * 4);
-            pf[sl][j][1] = ld_f32(rf, zl * 4 + 4);
+            pf[sl][j][0] = ld_f32(rf[j], f0);
+            pf[sl][j][1] = ld_f32(rf[j], f1);
             if constexpr (KIND == TF_KIND_FIELD) {
-                pc[sl][j][0] = ld_f32(rc, zl * 4);
-                pc[sl][j][1] = ld_f32(rc, zl * 4 + 4);
+                pc[sl][j][0] = ld_f32(rc[j], c0);
+                pc[sl][j][1] = ld_f32(rc[j], c1);
             } else {
-                pc[sl][j][0] = ld_u8(rc, zl);
-                pc[sl][j][1] = ld_u8(rc, zl + 1);
+                pc[sl][j][0] = ld_u8(rc[j], c0);
+                pc[sl][j][1] = ld_u8(rc[j], c1);
             }
         }
     };
-    auto commit_load = [&](int p, auto slot_c) {
+    auto commit_load = [&](int slot, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
-        float* P = sF + (p % NP) * PS;
-        CT* CP = sC + (p % NP) * PS;
+        float* P = sF + slot * PS;
+        CT* CP = sC + slot * PS;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int o = (wave + j * NW) * kRowLen + lane;
@@ -559,18 +581,18 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     };
 
     // ---- one half-sweep of one plane ----
-    auto stage = [&](int h, int p) {
+    auto stage = [&](int h, int p, int sp, int spm, int spp) {
         const int grow = H - 1 - h;
         if (p < max(x0 - grow, 1) || p >= min(x1 + grow, v.X + 1)) return;
         const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
         const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
         const int colour = colour0 ^ (h & 1);
-        float* P = sF + (p % NP) * PS;
-        const float* Pm = sF + ((p - 1) % NP) * PS;
-        const float* Pp = sF + ((p + 1) % NP) * PS;
-        const CT* CP = sC + (p % NP) * PS;
-        const CT* CPm = sC + ((p - 1) % NP) * PS;
-        const CT* CPp = sC + ((p + 1) % NP) * PS;
+        float* P = sF + sp * PS;
+        const float* Pm = sF + spm * PS;
+        const float* Pp = sF + spp * PS;
+        const CT* CP = sC + sp * PS;
+        const CT* CPm = sC + spm * PS;
+        const CT* CPp = sC + spp * PS;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             const int r = wave + j * NW;
@@ -630,35 +652,31 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
         }
     };
 
-    auto store_plane = [&](int q) {
-        // every lane stores; lanes outside the owned tile (plane, row or z)
-        // use an out-of-range offset, which drops the write
+    auto store_plane = [&](int q, int sq) {
         const bool qok = q >= x0 && q < x1;
-        const float* P = sF + (qok ? q % NP : 0) * PS;
-        const bool own0 = zl >= z0 && zl < z1, own1 = zl + 1 >= z0 && zl + 1 < z1;
+        const float* P = sF + sq * PS;
+        const int bq = q * plane_f + zl * 4;
+        const int o0 = (qok && own0) ? bq : kOff, o1 = (qok && own1) ? bq + 4 : kOff;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const int r = wave + j * NW;
-            const int y = ey0 + r;
-            const bool ok = qok && y >= y0 && y < y1;
-            const size_t row = ok ? (size_t)q * v.sX + (size_t)y * v.sY : 0;
-            const rsrc_t rd = make_rsrc(D + row, ok ? (uint32_t)v.sY * 4u : 0u);
-            const int lo = r * kRowLen + lane;
-            st_f32(P[lo], rd, own0 ? zl * 4 : -1);
-            st_f32(P[lo + kRowPairs], rd, own1 ? zl * 4 + 4 : -1);
+            const int lo = (wave + j * NW) * kRowLen + lane;
+            st_f32(P[lo], rd[j], o0);
+            st_f32(P[lo + kRowPairs], rd[j], o1);
         }
     };
 
     const int S0 = max(x0 - (H - 1), 1);
     const int S1 = (x1 - 1) + (H - 1);
+    auto wrap = [](int sl) { return sl >= NP ? sl - NP : (sl < 0 ? sl + NP : sl); };
     using I0 = std::integral_constant<int, 0>;
     // planes S0-1 .. S0+1 go straight to LDS; S0+2 .. S0+kPrefetch+1 are put in flight
+    const int b0 = (S0 - 1) % NP;  // LDS slot of plane S0-1
     issue_load(S0 - 1, I0{});
-    commit_load(S0 - 1, I0{});
+    commit_load(b0, I0{});
     issue_load(S0, I0{});
-    commit_load(S0, I0{});
+    commit_load(wrap(b0 + 1), I0{});
     issue_load(S0 + 1, I0{});
-    commit_load(S0 + 1, I0{});
+    commit_load(wrap(b0 + 2), I0{});
     static_for<2, kPrefetch + 2>([&](auto d_c) {
         constexpr int d = decltype(d_c)::value;
         issue_load(S0 + d, std::integral_constant<int, d % kPrefetch>{});
@@ -668,21 +686,25 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     // step s: put plane s+1+kPrefetch in flight, run half-sweep h on plane
     // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS.
     // Steps past S1 (to complete the last group of kPrefetch) do no work
-    // but keep the memory instructions unconditional.
+    // but keep the memory instructions unconditional.  bs = LDS slot of plane s.
+    int bs0 = wrap(b0 + 1);
     for (int s0 = S0; s0 <= S1; s0 += kPrefetch) {
         static_for<0, kPrefetch>([&](auto u_c) {
             constexpr int u = decltype(u_c)::value;
             const int s = s0 + u;
+            const int bs = wrap(bs0 + u);
             issue_load(s + 1 + kPrefetch, std::integral_constant<int, (u + 1) % kPrefetch>{});
 #pragma unroll
             for (int h = 0; h < H; ++h) {
-                stage(h, s - h);
+                const int sp = wrap(bs - h);
+                stage(h, s - h, sp, wrap(sp - 1), wrap(sp + 1));
                 if (h + 1 < H) __syncthreads();
             }
-            store_plane(s - (H - 1));
-            commit_load(s + 2, std::integral_constant<int, (u + 2) % kPrefetch>{});
+            store_plane(s - (H - 1), wrap(bs - (H - 1)));
+            commit_load(wrap(bs + 2), std::integral_constant<int, (u + 2) % kPrefetch>{});
             __syncthreads();
         });
+        bs0 = wrap(bs0 + kPrefetch);
     }
 }
 
@@ -796,6 +818,9 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
         if (tile->tx > 0) t.tx = tile->tx;
     }
     if (t.tx < 1 || t.tx > 65536) return fail(-1, "invalid tile tx=%d", t.tx);
+    if ((long long)(X + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31))
+        return fail(-1, "padded volume of %lld bytes: the sweep addresses at most 2 GiB per volume",
+                    (long long)(X + 2) * (Y + 2) * (Z + 2) * 4);
     SweepArgs a;
     a.src = src; a.dst = dst; a.codes = codes; a.hm = hm_lut; a.ncls = n_classes;
     a.v = make_vol(X, Y, Z);

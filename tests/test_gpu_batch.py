@@ -52,6 +52,19 @@ def test_batch_equals_solo_solves(use_graph, H):
         assert np.array_equal(b.field(v).cpu().numpy(), s.field[0].cpu().numpy()), v
 
 
+def test_batch_volumes_match_oracle():
+    """Independent of the solo GPU path: two volumes of a batch against the oracle."""
+    from oracle.restate import OracleSolver
+
+    imgs = _volumes(seed=5, n=3)
+    b = tau.BatchSolver(imgs)
+    b.solve()
+    for v in (0, 2):
+        o = OracleSolver(imgs[v]).solve()
+        assert b.iters[v] == o.iter
+        assert np.array_equal(b.field(v).cpu().numpy(), o.field.numpy()[0])
+
+
 def test_batch_iteration_limit_and_reset():
     imgs = _volumes(seed=11, n=4)
     b = tau.BatchSolver(imgs)
