@@ -27,6 +27,11 @@ Other workloads (not part of the default line):
                           solve and per-phase dD_eff/dD (one gradient pass)
   --config field128x64    64 volumes of 128^3 with k = sigmoid(4 z): batched
                           forward solve and per-voxel dD_eff/dk
+  --config hbm_copy       device-to-device copy of one 512^3 float32 field
+                          (537 MB read + 537 MB written): the achievable-
+                          bandwidth yardstick, and the kernel whose
+                          FETCH_SIZE / WRITE_SIZE counters calibrate the
+                          sweep kernel's measured HBM traffic
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
                        [--config solver512|batch256|multiphase320|field128x64]
@@ -88,7 +93,7 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="solver512",
-                    choices=["solver512", "batch256", "multiphase320", "field128x64"])
+                    choices=["solver512", "batch256", "multiphase320", "field128x64", "hbm_copy"])
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -108,6 +113,8 @@ def main(argv=None) -> int:
         return bench_multiphase(args)
     if args.config == "field128x64":
         return bench_field(args)
+    if args.config == "hbm_copy":
+        return bench_copy(args)
 
     t0 = time.perf_counter()
     img = workloads.smoothed_binary(args.size, 0.35, 4.0, 0)
@@ -191,8 +198,11 @@ def main(argv=None) -> int:
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if t.get("half_sweeps_per_pass") == H:
+        if t.get("half_sweeps_per_pass") == H and (s.tile is None or list(s.tile) == t.get("tile")):
             out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
+            out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch, "
+                                                 "profiles/k_sweep_solver_%d_traffic.md"
+                                                 % (t["fetch_factor"], t["write_factor"], args.size))
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_iters} oracle iterations at {args.size}^3 ...")
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_iters)
@@ -208,6 +218,34 @@ def _line(metric, value, unit, steps, warmup, ms, config, higher=True, dtype="f3
            "config": config}
     out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def bench_copy(args) -> int:
+    """Copy of a --size^3 float32 field on the device, timed with HIP events."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    n = args.size ** 3
+    src = torch.rand(n, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(max(args.warmup, 1)):
+        dst.copy_(src)
+    reps = 20 * args.steps
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize(dev)
+    ms = a.elapsed_time(b) / reps
+    assert torch.equal(dst, src)
+    gbs = 2 * 4 * n / (ms / 1e3) / 1e9
+    log(f"copy {4 * n / 1e6:.0f} MB: {ms:.4f} ms -> {gbs:.0f} GB/s (read + write)")
+    _line("device copy GB/s (read + write)", round(gbs, 1), "GB/s", reps, args.warmup, round(ms, 5),
+          {"workload": f"torch copy_ of {n} float32 ({4 * n} B read, {4 * n} B written)"},
+          bytes_read=4 * n, bytes_written=4 * n)
+    return 0
 
 
 def bench_batch(args) -> int:

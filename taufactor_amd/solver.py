@@ -35,7 +35,7 @@ __all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "M
            "PeriodicMultiPhaseSolver", "FieldSolver"]
 
 # half-sweeps fused per HBM pass unless a solver is told otherwise
-DEFAULT_HALF_SWEEPS_PER_PASS = 4
+DEFAULT_HALF_SWEEPS_PER_PASS = 2
 # reference check cadence (taufactor/taufactor.py:184) and tau tolerance (:148)
 CHECK_EVERY = 100
 TAU_TOL = 2e-3
