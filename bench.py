@@ -9,11 +9,15 @@ re-packs it on the device).
 
 value       GVoxel-updates/s = voxels x (iterations/2) x steps / time
             (one voxel-update = one voxel through both colours)
-roofline    the sweep kernel (k_sweep): algorithmic HBM bytes per launch =
-            9 B x voxels (4 B read + 4 B write of the float32 field, 1 B code,
-            for however many half-sweeps the launch fuses), divided by the
-            kernel's mean duration measured with HIP events on its stream
-            during the timed steps; peak 8 TB/s
+roofline    the sweep kernel (k_sweep): algorithmic bytes per launch =
+            9 B per voxel-update (4 B read + 4 B write of the float32 field,
+            1 B code: one unblocked pass per voxel-update) x the voxel-updates
+            one launch performs (voxels x H/2 for H fused half-sweeps),
+            divided by the kernel's mean duration measured with HIP events on
+            its stream during the timed steps; peak 8 TB/s.  Temporal
+            blocking lets this exceed the bytes actually moved: `traffic`
+            holds the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
+            WRITE_SIZE, calibrated), when a summary for this H and tile exists
 cpu_baseline  the CPU oracle (an op-for-op torch restatement of the
             reference loop) on this host, on a bounded sample: a fixed number
             of iterations of the same 512^3 volume, with one convergence check
@@ -152,7 +156,9 @@ def main(argv=None) -> int:
     launches = sum(n for _, _, n in events)
     nvox = img.size
     per_launch_s = sweep_ms / 1e3 / launches
-    achieved = BYTES_PER_VOXEL_PASS * nvox / per_launch_s / 1e9
+    # every launch of the timed solves fuses H half-sweeps unless 100 % H != 0
+    updates_per_launch = nvox * sum(iters) / 2 / launches
+    achieved = BYTES_PER_VOXEL_PASS * updates_per_launch / per_launch_s / 1e9
     gvus = nvox * sum(iters) / 2 / elapsed / 1e9
     log(f"{args.steps} solves, iterations {iters}, {elapsed:.3f} s; sweep kernel mean "
         f"{per_launch_s * 1e3:.3f} ms over {launches} launches -> {achieved:.0f} GB/s")
@@ -189,7 +195,7 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "bytes_per_launch_algorithmic": BYTES_PER_VOXEL_PASS * nvox,
+            "bytes_per_launch_algorithmic": int(BYTES_PER_VOXEL_PASS * updates_per_launch),
             "mean_launch_ms": round(per_launch_s * 1e3, 4),
         },
         "cpu_baseline": None,
@@ -198,7 +204,7 @@ def main(argv=None) -> int:
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if t.get("half_sweeps_per_pass") == H and (s.tile is None or list(s.tile) == t.get("tile")):
+        if t.get("half_sweeps_per_pass") == H and t.get("tile") == (list(s.tile) if s.tile else "default"):
             out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
             out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch, "
                                                  "profiles/k_sweep_solver_%d_traffic.md"

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 2
+#define TF_ABI_VERSION 3
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -46,13 +46,22 @@ extern "C" {
 #define TF_KIND_FIELD 2
 
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
- * tile of (waves*rows_per_wave - 2*n_half) rows in y by (128 - 2*n_half)
- * voxels in z along tx planes of x; one wavefront per row.  Compiled shapes
- * (waves x rows_per_wave): 8x2, 8x3, 16x1, 4x4.  Zero fields select the
- * defaults (8x2 for n_half <= 3 and for TF_KIND_FIELD, 8x3 above; tx 64). */
+ * tile of (waves*rows_per_wave - 2*n_half) rows in y by TZ voxels in z
+ * (TZ = 120 for n_half <= 4, 112 for 5 and 6) along tx planes of x; one
+ * wavefront per row.  Compiled shapes (waves x rows_per_wave): 8x2, 4x4,
+ * 12x2, 8x4, 16x2.  Zero fields select the defaults: 16x2 for one-byte
+ * codes with n_half <= 4 and Y >= 64, else 8x2 (8x2 for batches); tx 64,
+ * shortened (>= 8) for small volumes to keep about 512 workgroups. */
 typedef struct tf_tile {
     int waves, rows_per_wave, tx;
 } tf_tile;
+
+/* Readable bytes tf_sweep and tf_batch_sweep require before the first and
+ * after the last element of every field buffer (and of the per-voxel
+ * diffusivity array of TF_KIND_FIELD); they must hold finite float32
+ * values (zeros).  The sweep reads the field in 16-byte quads that start up
+ * to 3 voxels before z = 0. */
+#define TF_FIELD_SLACK_BYTES 16
 
 /* ABI version of the loaded library (TF_ABI_VERSION). No reference
  * counterpart: lets the Python loader refuse a stale build. */
@@ -104,6 +113,19 @@ int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
  * and a one-half-sweep tf_sweep.                                          */
 int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream);
 
+/* Size in bytes of the sweep-code layout of one volume: (X+2)*(Y+2) rows of
+ * tf_sweep_code_row(Z) = roundup(Z+5, 4) bytes; byte z+3 of row (x, y)
+ * holds the code of padded voxel (x, y, z) for z in [0, Z+1], every other
+ * byte is 0.  Rows are 4-byte aligned so that the sweep reads four codes
+ * per lane with one aligned load.  No reference counterpart (the reference
+ * keeps a float32 prefactor array, taufactor/taufactor.py:402-410).      */
+long long tf_sweep_codes_bytes(int X, int Y, int Z);
+
+/* Repack codes (layout above, B volumes) into the sweep-code layout
+ * (B consecutive volumes of tf_sweep_codes_bytes each; 4-byte aligned).  */
+int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int B, int X, int Y, int Z,
+                        void* stream);
+
 /* n_half over-relaxed red-black half-sweeps (reference iterations), the
  * first one on colour first_colour, fused in one HBM pass: reads src,
  * writes every interior voxel of dst (ghosts of dst are left untouched and
@@ -113,6 +135,10 @@ int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream);
  * MultiPhaseSolver.sum_weighted_neighbours (:606-613).  Arithmetic is the
  * reference's float32 operation order, without contraction, so the result
  * equals n_half reference iterations value for value.
+ *   src, dst field buffers with TF_FIELD_SLACK_BYTES of slack
+ *   codes    TF_KIND_SOLVER / TF_KIND_MULTIPHASE: the sweep-code layout
+ *            (tf_pack_sweep_codes); TF_KIND_FIELD: the padded per-voxel
+ *            diffusivity with TF_FIELD_SLACK_BYTES of slack
  *   hm_lut   device float [n_classes*n_classes] face conductances
  *            (TF_KIND_MULTIPHASE), ignored for TF_KIND_SOLVER
  *   omega    the over-relaxation factor rounded to float32
@@ -184,6 +210,7 @@ int tf_grad_classes(const float* field, const uint8_t* codes, const float* d_cls
  * packed buffers; layout of each volume as above (B = 1). */
 typedef struct tf_volume {
     long long offset;   /* first element of the padded volume in field / codes */
+    long long code_offset;  /* first byte of the volume in the packed sweep codes */
     int X, Y, Z;
     float omega;        /* over-relaxation factor rounded to float32 */
     int sum_offset;     /* first entry in the packed layer-sum arrays (X per volume) */
@@ -209,9 +236,10 @@ int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_tile* tile, in
 /* n_half half-sweeps on every active volume of a batch, one launch.  All
  * active volumes read field_a (src_buffer 0) or field_b (1) and write the
  * other.  tile_vol[t] = volume of tile t (tiles of volume v are
- * tile_first .. tile_first + tf_sweep_tile_count - 1).  Same arithmetic as
- * tf_sweep.                                                              */
-int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* codes,
+ * tile_first .. tile_first + tf_sweep_tile_count - 1).  sweep_codes holds
+ * each volume's sweep-code layout at its code_offset; field_a and field_b
+ * need TF_FIELD_SLACK_BYTES of slack.  Same arithmetic as tf_sweep.      */
+int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* sweep_codes,
                    const tf_volume* vols, const int* tile_vol, int n_tiles,
                    const tf_volume_state* state, int src_buffer, int first_colour,
                    int n_half, const tf_tile* tile, void* stream);

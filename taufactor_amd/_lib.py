@@ -11,13 +11,15 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
 KIND_FIELD = 2
 SETUP_PERIODIC_YZ = 1
 MAX_FUSED_HALF_SWEEPS = 6
+# readable bytes the sweep needs before and after each field buffer (TF_FIELD_SLACK_BYTES)
+FIELD_SLACK_BYTES = 16
 
 _c_int = ctypes.c_int
 _c_float = ctypes.c_float
@@ -37,6 +39,8 @@ SIGNATURES = {
     "tf_setup_multiphase": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp,
                                      _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_wrap_ghosts": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_sweep_codes_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int]),
+    "tf_pack_sweep_codes": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                           _c_float, _c_int, _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_sweep_half_inplace": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,
@@ -58,7 +62,7 @@ SIGNATURES = {
 # numpy mirrors of the batch structs (layout checked in tests/test_abi.py)
 import numpy as _np  # noqa: E402
 
-VOLUME_DTYPE = _np.dtype([("offset", _np.int64), ("X", _np.int32), ("Y", _np.int32), ("Z", _np.int32),
+VOLUME_DTYPE = _np.dtype([("offset", _np.int64), ("code_offset", _np.int64), ("X", _np.int32), ("Y", _np.int32), ("Z", _np.int32),
                           ("omega", _np.float32), ("sum_offset", _np.int32), ("tile_first", _np.int32),
                           ("spans", _np.int32), ("reserved", _np.int32), ("D_mean", _np.float64)])
 STATE_DTYPE = _np.dtype([("active", _np.int32), ("converged", _np.int32), ("iters", _np.int32),
@@ -109,6 +113,29 @@ def ptr(t) -> int:
             raise ValueError("tensor must be contiguous")
         return t.data_ptr()
     return t.ctypes.data
+
+
+def slack_buffer(shape, dtype, device):
+    """A contiguous tensor of `shape` inside a zero-filled allocation with
+    FIELD_SLACK_BYTES before and after it (what tf_sweep needs of field
+    buffers and of the per-voxel diffusivity)."""
+    import torch
+    itemsize = torch.empty((), dtype=dtype).element_size()
+    pad = FIELD_SLACK_BYTES // itemsize
+    n = 1
+    for d in shape:
+        n *= int(d)
+    flat = torch.zeros(n + 2 * pad, dtype=dtype, device=device)
+    return flat[pad:pad + n].view(*shape)
+
+
+def sweep_code_buffer(B, X, Y, Z, device):
+    """Uninitialised uint8 buffer for B volumes of the sweep-code layout."""
+    import torch
+    n = load().tf_sweep_codes_bytes(X, Y, Z)
+    if n < 0:
+        check(int(n), "tf_sweep_codes_bytes")
+    return torch.empty(B * int(n), dtype=torch.uint8, device=device)
 
 
 def stream_handle(device) -> int:

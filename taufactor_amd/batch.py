@@ -58,12 +58,13 @@ class BatchSolver:
         self.device = Solver._init_device(device)
         self._lib = _lib.load()
         self.D_0 = D_0
-        self.half_sweeps_per_pass = 2
+        self.half_sweeps_per_pass = 4
         self.tile = None
         self.shapes = [tuple(int(d) for d in i.shape) for i in imgs]
         V = self.n_volumes = len(imgs)
         vols = np.zeros(V, dtype=_lib.VOLUME_DTYPE)
         off = 0
+        coff = 0
         moff = 0
         soff = 0
         mask_offsets = []
@@ -72,11 +73,13 @@ class BatchSolver:
                 raise ValueError(f"volume {v} of shape {(X, Y, Z)} is too large for a batch (2 GiB per volume)")
             w = 2 - math.pi / (1.5 * X) if omega is None else omega
             vols[v]["offset"] = off
+            vols[v]["code_offset"] = coff
             vols[v]["X"], vols[v]["Y"], vols[v]["Z"] = X, Y, Z
             vols[v]["omega"] = np.float32(w)
             vols[v]["sum_offset"] = soff
             mask_offsets.append(moff)
             off += (X + 2) * (Y + 2) * (Z + 2)
+            coff += int(self._lib.tf_sweep_codes_bytes(X, Y, Z))
             moff += X * Y * Z
             soff += X
         self._vols_host = vols
@@ -87,9 +90,10 @@ class BatchSolver:
             xinit = torch.cat([torch.linspace(-0.5 + 1 / (2 * X), 0.5 - 1 / (2 * X), X, dtype=torch.float32)
                                for (X, _, _) in self.shapes])
             self._xinit_dev = xinit.to(self.device)
-            self._fa = torch.empty(off, dtype=torch.float32, device=self.device)
-            self._fb = torch.empty(off, dtype=torch.float32, device=self.device)
+            self._fa = _lib.slack_buffer((off,), torch.float32, self.device)
+            self._fb = _lib.slack_buffer((off,), torch.float32, self.device)
             self._codes = torch.empty(off, dtype=torch.uint8, device=self.device)
+            self._sweep_codes = torch.empty(coff, dtype=torch.uint8, device=self.device)
             self._count = torch.empty(soff, dtype=torch.int32, device=self.device)
             self._flux = torch.zeros(soff, dtype=torch.float64, device=self.device)
             self._conc = torch.zeros(soff, dtype=torch.float64, device=self.device)
@@ -123,6 +127,10 @@ class BatchSolver:
                 _lib.ptr(self._codes) + o, _lib.ptr(self._fa) + 4 * o, _lib.ptr(self._fb) + 4 * o,
                 _lib.ptr(self._count) + 4 * s0, 1, X, Y, Z, 0, self._stream())
             _lib.check(st, "tf_setup_solver")
+            st = self._lib.tf_pack_sweep_codes(
+                _lib.ptr(self._codes) + o, _lib.ptr(self._sweep_codes) + int(self._vols_host[v]["code_offset"]),
+                1, X, Y, Z, self._stream())
+            _lib.check(st, "tf_pack_sweep_codes")
 
     def reset(self):
         """Back to the initial fields (volumes resident on the device)."""
@@ -153,7 +161,7 @@ class BatchSolver:
 
     def _pass(self, src_b, colour, h):
         p = self._plan
-        st = self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._codes),
+        st = self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes),
                                       _lib.ptr(self._vols_dev), _lib.ptr(self._tile_vol_dev), p["n_tiles"],
                                       _lib.ptr(self._state_dev), src_b, colour, h, p["tile"], self._stream())
         _lib.check(st, "tf_batch_sweep")

@@ -67,6 +67,13 @@ __host__ __device__ static inline Vol make_vol(int X, int Y, int Z) {
     return v;
 }
 
+// sweep-code layout: rows of sweep_code_row(Z) bytes (a multiple of 4),
+// byte z+3 of a row = code of padded voxel z (z in [0, Z+1]), other bytes 0
+__host__ __device__ inline int sweep_code_row(int Z) { return ((Z + 5 + 3) / 4) * 4; }
+__host__ __device__ inline long long sweep_code_volume(int X, int Y, int Z) {
+    return (long long)(X + 2) * (Y + 2) * sweep_code_row(Z);
+}
+
 static int check_dims(int B, int X, int Y, int Z) {
     if (B < 1 || X < 1 || Y < 1 || Z < 1)
         return fail(-1, "invalid volume shape B=%d X=%d Y=%d Z=%d", B, X, Y, Z);
@@ -304,6 +311,45 @@ extern "C" int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conduc
                            B, X, Y, Z, flags, stream);
 }
 
+// one thread per 4-byte word of the sweep-code layout (rows are whole words)
+__global__ void k_pack_sweep_codes(const uint8_t* __restrict__ codes, uint32_t* __restrict__ out,
+                                   Vol v, int row_c, long long n_words) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n_words) return;
+    const long long byte0 = gid * 4;
+    const long long row = byte0 / row_c;  // row index over B*(X+2)*(Y+2)
+    const int k0 = (int)(byte0 - row * row_c);
+    const uint8_t* src = codes + row * (long long)v.sY;
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int z = k0 + i - 3;
+        const uint32_t c = (z >= 0 && z <= v.Z + 1) ? src[z] : 0u;
+        w |= c << (8 * i);
+    }
+    out[gid] = w;
+}
+
+extern "C" long long tf_sweep_codes_bytes(int X, int Y, int Z) {
+    if (X < 1 || Y < 1 || Z < 1) return fail(-1, "invalid shape %dx%dx%d", X, Y, Z);
+    return sweep_code_volume(X, Y, Z);
+}
+
+extern "C" int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int B, int X, int Y,
+                                   int Z, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!codes || !sweep_codes) return fail(-1, "null pointer argument");
+    if (reinterpret_cast<uintptr_t>(sweep_codes) & 3) return fail(-1, "sweep_codes must be 4-byte aligned");
+    const long long n_words = (long long)B * sweep_code_volume(X, Y, Z) / 4;
+    const long long blocks = (n_words + 255) / 256;
+    if (blocks >= (1LL << 31)) return fail(-1, "volume too large");
+    hipLaunchKernelGGL(k_pack_sweep_codes, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       codes, reinterpret_cast<uint32_t*>(sweep_codes), make_vol(X, Y, Z),
+                       sweep_code_row(Z), n_words);
+    TF_LAUNCH_CHECK("k_pack_sweep_codes");
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 // the update of one voxel (shared by both sweep kernels)
 // ---------------------------------------------------------------------------
@@ -418,7 +464,7 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
 // ---------------------------------------------------------------------------
 // A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
 // coordinates) and streams it plane by plane along x through a ring of H+3
-// planes in LDS, each plane extended by H voxels in y and z.  Half-sweep h
+// planes in LDS, each plane extended by a halo in y and z.  Half-sweep h
 // (h = 0..H-1, colour first_colour^(h&1)) runs on plane s-h at step s, on
 // the tile grown by H-1-h voxels, so that after the last half-sweep the
 // owned tile holds exactly the values of H successive reference iterations.
@@ -426,14 +472,25 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
 // so reading src and writing dst (no in-place hazard between workgroups
 // that recompute each other's halos) gives the in-place result.
 //
-// Mapping: one wavefront per tile row (y), one lane per pair of voxels in z,
-// so a row is 128 voxels of z (tile interior TZ = 128 - 2H) and the colour
-// pattern of a row is uniform across the wave.  An LDS row stores the 64
-// even-z voxels, then the 64 odd-z voxels: every LDS access of a half-sweep
-// is lane-contiguous (no bank conflicts) at a constant offset from one base.
-// Each wave owns RPW rows (interleaved), NW waves make EY = NW*RPW rows
-// (tile interior TY = EY - 2H).  The next plane is prefetched into registers
-// while the half-sweeps of the current step run.
+// Compute mapping: one wavefront per tile row (y), one lane per pair of
+// voxels in z, so an LDS row is a window of 128 voxels of z starting at ew
+// and the colour pattern of a row is uniform across the wave.  An LDS row
+// stores the first voxel of each pair (64), then the second (64): every LDS
+// access of a half-sweep is lane-contiguous (no bank conflicts).  Each wave
+// owns RPW rows (interleaved, NW even so that all rows of a wave share their
+// colour parity), NW waves make EY = NW*RPW rows (tile interior TY = EY-2H).
+// The window starts ML >= H voxels before the tile, with ML and the tile
+// width TZ multiples of 4 so that z0 and ew are 1 mod 4.
+//
+// Memory mapping: a plane of the tile is read with one 16-byte load per lane
+// covering two rows (lanes 0-31: row 2m, lanes 32-63: row 2m+1, four z
+// each), codes with one 4-byte load per lane from the aligned sweep-code
+// layout, and written back with one 8-byte store per lane and row (one pair
+// each).  The x loop is unrolled by the ring length NP, and plane q lives in
+// LDS slot and register slot (q - S0 + 1) mod NP: every slot is a
+// compile-time constant, so each LDS access is one instruction with an
+// immediate offset from one of a few per-lane base addresses.  Planes are
+// loaded NP-1 steps ahead of their use.
 //
 // Workgroups are numbered so that consecutive tiles land on the same XCD
 // (hardware dispatch sends workgroup i to XCD i mod 8): neighbouring tiles
@@ -460,17 +517,27 @@ __device__ __forceinline__ void static_for(F&& f) {
     }
 }
 
-constexpr int kPrefetch = 3;           // planes in flight per workgroup
-constexpr int kRowPairs = 64;          // lanes per row
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kRowPairs = 64;           // lanes per row
 constexpr int kRowLen = 2 * kRowPairs;  // voxels per LDS row
+constexpr int kLdsPad = kRowLen * 4;    // bytes before the plane ring: keeps the row-below
+                                        // bias of the base addresses non-negative
+constexpr int kSlackBytes = TF_FIELD_SLACK_BYTES;
 
 template <int H, int NW, int RPW>
 struct SweepShape {
     static constexpr int EY = NW * RPW;
     static constexpr int TY = EY - 2 * H;
-    static constexpr int TZ = kRowLen - 2 * H;
-    static constexpr int NP = H + 3;      // planes in the LDS ring
-    static constexpr int PS = EY * kRowLen;  // floats (and code bytes) per plane
+    static constexpr int ML = 4 * ((H + 3) / 4);                // z0 - ew
+    static constexpr int TZ = ((kRowLen - ML - H) / 4) * 4;     // owned voxels of z
+    static constexpr int NP = H + 3;         // planes in the LDS ring and in the register ring
+    static constexpr int PS = EY * kRowLen;  // floats (and codes) per plane
+    static constexpr int RPL = RPW / 2;      // row pairs loaded per wave and plane
 };
 
 // per-voxel auxiliary data: a one-byte code (Solver, MultiPhase) or the
@@ -485,6 +552,13 @@ __device__ __forceinline__ float hmean(float a, float b) {
     return den > 0.0f ? ((2.0f * a) * b) / den : 0.0f;
 }
 
+// (1/d, d) per Solver code for the sweep.  Code 0 (factor inf) uses (0, 0):
+// the sequence of div_by_count then yields a zero whose sign can differ from
+// s/inf only when s is -0, and c + w*(q - c) is the same float for either
+// zero q whatever c is (c = +-0 gives +0, c != 0 gives c - w*c).
+__constant__ float c_rd[18] = {0.0f, 0.0f, 1.0f, 1.0f, 0.5f, 2.0f, 1.0f / 3.0f, 3.0f, 0.25f, 4.0f,
+                               0.2f, 5.0f, 1.0f / 6.0f, 6.0f, 1.0f / 7.0f, 7.0f, 0.125f, 8.0f};
+
 template <int H, int KIND, int NW, int RPW>
 __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
                                            const code_t<KIND>* __restrict__ Cg,
@@ -492,219 +566,280 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
                                            float w, int colour0, int TX, int tz, int ty, int tx) {
     using Sh = SweepShape<H, NW, RPW>;
     using CT = code_t<KIND>;
-    constexpr int EY = Sh::EY, TY = Sh::TY, TZ = Sh::TZ, NP = Sh::NP, PS = Sh::PS;
+    constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NP = Sh::NP, PS = Sh::PS, RPL = Sh::RPL;
     static_assert(TY >= 1, "tile has no interior rows");
+    static_assert(NW % 2 == 0, "the rows of a wave must share their colour parity");
+    static_assert(RPW % 2 == 0, "rows are loaded in pairs");
     extern __shared__ __align__(16) unsigned char smem[];
-    float* sF = reinterpret_cast<float*>(smem);
+    float* sF = reinterpret_cast<float*>(smem + kLdsPad);
     CT* sC = reinterpret_cast<CT*>(sF + NP * PS);
     float* sT = reinterpret_cast<float*>(sC + NP * PS);
-    (void)EY;
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
     const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
-    const int ey0 = y0 - H, ez0 = z0 - H;
-    const int zl = ez0 + 2 * lane;  // z of this lane's even element
+    const int ey0 = y0 - H;
+    const int ew = z0 - ML;         // z of window element 0; ew = 1 mod 4
+    const int zl = ew + 2 * lane;   // z of the first voxel of this lane's pair
     const int lx1 = min(x1 + H, v.X + 2);
 
     if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 16) sT[threadIdx.x] = threadIdx.x < 9 ? c_rcp[threadIdx.x] : 0.0f;
+        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
     } else if (KIND == TF_KIND_MULTIPHASE) {
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
     }
 
-    // ---- plane loads: global -> registers (kPrefetch planes in flight) -> LDS ----
-    // Register slot of plane q is (q - S0) mod kPrefetch; every use below has
-    // a compile-time slot, so the ring stays in registers.
-    //
-    // Global memory is addressed through one buffer descriptor per tile row,
-    // built once: base = that row in plane 0, record count = the rest of the
-    // volume (0 for rows outside [0, Y+1], and for stores outside the owned
-    // rows).  The lane offset is plane * plane_bytes + z * elem_bytes.  Lanes
-    // whose z is outside [0, Z+1], planes past the tile's last plane and
-    // stores outside the owned tile use an out-of-range lane offset: loads
-    // read 0, stores are dropped, no memory traffic.  No branches around
-    // memory instructions, so the waitcnt pass can keep kPrefetch planes in
-    // flight.
+    // ---- global memory ----
+    // Loads: one descriptor per array for the whole tile.  The field (and
+    // the per-voxel diffusivity) descriptor starts kSlackBytes before the
+    // volume and spans it plus the slack on both sides, so the window's
+    // first quad (z = ew >= -3) is never at a negative offset; the lane
+    // offset holds row and z, the wave-uniform offset the plane.  Rows
+    // outside [0, Y+1] and planes past the tile's last plane use a lane
+    // offset past any record count: loads read 0, no memory traffic.  The
+    // range check is per dword, so quads running past the volume read 0
+    // there.
+    // Stores: one descriptor per row and plane, based at z = 0 of that row
+    // with record count 4*z1 (0 outside the owned rows and planes), so a
+    // pair straddling z1 = Z+1 stores only its first voxel; pairs before z0
+    // use the out-of-range lane offset.
+    // No branches around memory instructions, so the waitcnt pass keeps NP-1
+    // planes in flight.
     constexpr int kOff = 0x7fffff00;  // lane offset past any record count
-    const int plane_f = v.sX * 4, plane_c = v.sX * (int)sizeof(CT);
-    const int vol_f = (int)(v.sB * 4), vol_c = (int)(v.sB * (long long)sizeof(CT));
-    rsrc_t rf[RPW], rc[RPW], rd[RPW];
-#pragma unroll
-    for (int j = 0; j < RPW; ++j) {
-        const int y = ey0 + wave + j * NW;
-        const bool rok = y >= 0 && y <= v.Y + 1;
-        const bool own = y >= y0 && y < y1;
-        const int yr = rok ? y : 0;
-        rf[j] = make_rsrc(S + (size_t)yr * v.sY, rok ? (uint32_t)(vol_f - yr * v.sY * 4) : 0u);
-        rc[j] = make_rsrc(Cg + (size_t)yr * v.sY,
-                          rok ? (uint32_t)(vol_c - yr * v.sY * (int)sizeof(CT)) : 0u);
-        rd[j] = make_rsrc(D + (size_t)yr * v.sY, own ? (uint32_t)(vol_f - yr * v.sY * 4) : 0u);
+    const int plane_f = v.sX * 4;
+    const int vol_f = (int)(v.sB * 4);
+    const int half = lane >> 5, qd = lane & 31;
+    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
+                                (uint32_t)vol_f + 2u * kSlackBytes);
+    int vf[RPL], vc[RPL];
+    rsrc_t rC;
+    int plane_c;
+    if constexpr (KIND == TF_KIND_FIELD) {
+        plane_c = plane_f;
+        rC = make_rsrc(reinterpret_cast<const char*>(Cg) - kSlackBytes,
+                       (uint32_t)vol_f + 2u * kSlackBytes);
+    } else {
+        const int row_c = sweep_code_row(v.Z);
+        plane_c = (v.Y + 2) * row_c;
+        rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
     }
-    const bool zok0 = zl >= 0 && zl <= v.Z + 1, zok1 = zl + 1 >= 0 && zl + 1 <= v.Z + 1;
-    const bool own0 = zl >= z0 && zl < z1, own1 = zl + 1 >= z0 && zl + 1 < z1;
-
-    float pf[kPrefetch][RPW][2];
-    std::conditional_t<KIND == TF_KIND_FIELD, float, uint32_t> pc[kPrefetch][RPW][2];
-    auto issue_load = [&](int p, auto slot_c) {
-        constexpr int sl = decltype(slot_c)::value;
-        const bool pok = p < lx1;
-        const int bf = p * plane_f + zl * 4, bc = p * plane_c + zl * (int)sizeof(CT);
-        const int f0 = (pok && zok0) ? bf : kOff, f1 = (pok && zok1) ? bf + 4 : kOff;
-        const int c0 = (pok && zok0) ? bc : kOff, c1 = (pok && zok1) ? bc + (int)sizeof(CT) : kOff;
 #pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            pf[sl][j][0] = ld_f32(rf[j], f0);
-            pf[sl][j][1] = ld_f32(rf[j], f1);
+    for (int i = 0; i < RPL; ++i) {
+        const int y = ey0 + 2 * (wave + i * NW) + half;
+        const bool rok = y >= 0 && y <= v.Y + 1;
+        vf[i] = rok ? kSlackBytes + (y * v.sY + ew + 4 * qd) * 4 : kOff;
+        if constexpr (KIND == TF_KIND_FIELD)
+            vc[i] = vf[i];
+        else
+            vc[i] = rok ? y * sweep_code_row(v.Z) + (ew + 3) + 4 * qd : kOff;
+    }
+
+    u32x4_t pf[NP][RPL];
+    std::conditional_t<KIND == TF_KIND_FIELD, u32x4_t, uint32_t> pc[NP][RPL];
+    auto issue_load = [&](int p, auto reg_c) {
+        constexpr int rg = decltype(reg_c)::value;
+        const bool pok = p < lx1;
+        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            pf[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rF, pok ? vf[i] : kOff, sf, 0);
+            if constexpr (KIND == TF_KIND_FIELD)
+                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rC, pok ? vc[i] : kOff, sc, 0);
+            else
+                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b32(rC, pok ? vc[i] : kOff, sc, 0);
+        }
+    };
+    // this lane's first quad slot (pairs 2qd, 2qd+1) in row 2*wave+half of slot 0
+    float* const Fl = sF + (2 * wave + half) * kRowLen + 2 * qd;
+    CT* const Cl = sC + (2 * wave + half) * kRowLen + 2 * qd;
+    auto commit_load = [&](auto slot_c, auto reg_c) {
+        constexpr int sl = decltype(slot_c)::value, rg = decltype(reg_c)::value;
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+            const int o = sl * PS + 2 * i * NW * kRowLen;
+            const u32x4_t q = pf[rg][i];
+            *reinterpret_cast<u32x2_t*>(Fl + o) = u32x2_t{q.x, q.z};               // first voxels
+            *reinterpret_cast<u32x2_t*>(Fl + o + kRowPairs) = u32x2_t{q.y, q.w};   // second voxels
             if constexpr (KIND == TF_KIND_FIELD) {
-                pc[sl][j][0] = ld_f32(rc[j], c0);
-                pc[sl][j][1] = ld_f32(rc[j], c1);
+                const u32x4_t k = pc[rg][i];
+                *reinterpret_cast<u32x2_t*>(Cl + o) = u32x2_t{k.x, k.z};
+                *reinterpret_cast<u32x2_t*>(Cl + o + kRowPairs) = u32x2_t{k.y, k.w};
             } else {
-                pc[sl][j][0] = ld_u8(rc[j], c0);
-                pc[sl][j][1] = ld_u8(rc[j], c1);
+                const uint32_t c = pc[rg][i];  // codes of z = 4qd+ew .. +3, byte 0 first
+                *reinterpret_cast<uint16_t*>(Cl + o) =
+                    (uint16_t)((c & 0xffu) | ((c >> 8) & 0xff00u));
+                *reinterpret_cast<uint16_t*>(Cl + o + kRowPairs) =
+                    (uint16_t)(((c >> 8) & 0xffu) | ((c >> 16) & 0xff00u));
             }
         }
     };
-    auto commit_load = [&](int slot, auto slot_c) {
+    // stores: rows of the compute mapping, one pair per lane
+    const float* Fw = sF + wave * kRowLen + lane;
+    const int zs = zl >= z0 ? zl * 4 : kOff;
+    bool own_row[RPW];
+    int ys[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j) {
+        const int y = ey0 + wave + j * NW;
+        own_row[j] = y >= y0 && y < y1;
+        ys[j] = own_row[j] ? y : 0;
+    }
+    auto store_plane = [&](int q, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
-        float* P = sF + slot * PS;
-        CT* CP = sC + slot * PS;
+        const bool qok = q >= x0 && q < x1;
+        const int qs = qok ? q : 0;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const int o = (wave + j * NW) * kRowLen + lane;
-            P[o] = pf[sl][j][0];
-            P[o + kRowPairs] = pf[sl][j][1];
-            CP[o] = (CT)pc[sl][j][0];
-            CP[o + kRowPairs] = (CT)pc[sl][j][1];
+            const int o = sl * PS + j * NW * kRowLen;
+            const rsrc_t r = make_rsrc(D + qs * v.sX + ys[j] * v.sY,
+                                       (qok && own_row[j]) ? (uint32_t)(4 * z1) : 0u);
+            const u32x2_t val{__builtin_bit_cast(uint32_t, Fw[o]),
+                              __builtin_bit_cast(uint32_t, Fw[o + kRowPairs])};
+            __builtin_amdgcn_raw_buffer_store_b64(val, r, zs, 0, 0);
+        }
+    };
+
+    // ---- per-tile bounds of each half-sweep ----
+    // planes [plo[h], plo[h] + pn[h]) and, as bit h*RPW+j of rowbits, the
+    // active rows.  In z every lane updates its active element: outside the
+    // half-sweep's region the result is never read by a later half-sweep of
+    // this pass nor stored, and on the z ghosts (value 0, code 0 / class 0 /
+    // k 0) the update returns the ghost's value.
+    int plo[H], pn[H];
+    int rowbits = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const int grow = H - 1 - h;
+        plo[h] = max(x0 - grow, 1);
+        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
+        const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const int y = ey0 + wave + j * NW;
+            rowbits |= (y >= ry0 && y < ry1) ? 1 << (h * RPW + j) : 0;
+        }
+    }
+
+    // The colour of (p, y, z) is (p+y+z+1)&1 and half-sweep h of step s
+    // updates colour colour0^(h&1) on plane s-h, so the active element of a
+    // lane's pair, act = (s + y + ew + 1 + colour0) & 1, is the same for all
+    // half-sweeps of a step and all rows of a wave; it alternates with s.
+    const int S0 = max(x0 - (H - 1), 1);
+    const int S1 = (x1 - 1) + (H - 1);
+    int A = (S0 + ey0 + wave + ew + 1 + colour0) & 1;  // act of the even steps of an iteration
+
+    // base addresses of the own element and its z neighbours for either
+    // parity of the step, biased one row down (the row below is at offset 0)
+    const float* Fo[2];
+    const float* Fp[2];
+    const float* Fm[2];
+    const CT* Co[2];
+    const CT* Cp[2];
+    const CT* Cm[2];
+    auto set_bases = [&](int a0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int a = a0 ^ q;
+            const int r0 = wave * kRowLen - kRowLen + lane;
+            const int io = r0 + (a ? kRowPairs : 0);
+            const int ip = r0 + (a ? 1 : kRowPairs);      // z+1: next pair's first / own second
+            const int im = r0 + (a ? 0 : kRowPairs - 1);  // z-1: own first / previous pair's second
+            Fo[q] = sF + io; Fp[q] = sF + ip; Fm[q] = sF + im;
+            Co[q] = sC + io; Cp[q] = sC + ip; Cm[q] = sC + im;
         }
     };
 
     // ---- one half-sweep of one plane ----
-    auto stage = [&](int h, int p, int sp, int spm, int spp) {
-        const int grow = H - 1 - h;
-        if (p < max(x0 - grow, 1) || p >= min(x1 + grow, v.X + 1)) return;
-        const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
-        const int rz0 = max(z0 - grow, 1), rz1 = min(z1 + grow, v.Z + 1);
-        const int colour = colour0 ^ (h & 1);
-        float* P = sF + sp * PS;
-        const float* Pm = sF + spm * PS;
-        const float* Pp = sF + spp * PS;
-        const CT* CP = sC + sp * PS;
-        const CT* CPm = sC + spm * PS;
-        const CT* CPp = sC + spp * PS;
+    auto stage = [&](auto h_c, auto q_c, auto sp_c, int p) {
+        constexpr int h = decltype(h_c)::value, q = decltype(q_c)::value;
+        constexpr int sp = decltype(sp_c)::value;
+        constexpr int spm = (sp + NP - 1) % NP, spp = (sp + 1) % NP;
+        if ((unsigned)(p - plo[h]) >= (unsigned)pn[h]) return;
+        float* const Fq = const_cast<float*>(Fo[q]);
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const int r = wave + j * NW;
-            const int y = ey0 + r;
-            if (y < ry0 || y >= ry1) continue;
-            // colour of (p,y,z) is (p+y+z+1)&1; the active element of the pair
-            const int act = ((p + y + ez0 + 1) & 1) ^ colour;
-            const int z = zl + act;
-            const bool ok = z >= rz0 && z < rz1;
-            const int o = r * kRowLen + act * kRowPairs + lane;        // own
-            const int oo = r * kRowLen + (act ^ 1) * kRowPairs + lane;  // other element, same pair
-            const int on = act ? oo + 1 : oo - 1;                       // other element, next pair
+            if (!((rowbits >> (h * RPW + j)) & 1)) continue;
+            const int o = sp * PS + j * NW * kRowLen + kRowLen;
+            const int om = spm * PS + j * NW * kRowLen + kRowLen;
+            const int op = spp * PS + j * NW * kRowLen + kRowLen;
             Nbr n;
-            n.c = P[o];
-            n.xm = Pm[o];
-            n.xp = Pp[o];
-            n.ym = P[o - kRowLen];
-            n.yp = P[o + kRowLen];
-            if (act) {
-                n.zm = P[oo];
-                n.zp = P[on];
-            } else {
-                n.zp = P[oo];
-                n.zm = P[on];
-            }
+            n.c = Fq[o];
+            n.xm = Fq[om];
+            n.xp = Fq[op];
+            n.ym = Fq[o - kRowLen];
+            n.yp = Fq[o + kRowLen];
+            n.zp = Fp[q][o];
+            n.zm = Fm[q][o];
             float out;
-            if (KIND == TF_KIND_SOLVER) {
-                const int code = CP[o];
-                const float r_ = sT[code];
-                const float d_ = code ? (float)code : __builtin_inff();
+            if constexpr (KIND == TF_KIND_SOLVER) {
+                const int code = Co[q][o];
+                const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
                 float s = n.xp + n.xm;
                 s = s + n.yp;
                 s = s + n.ym;
                 s = s + n.zp;
                 s = s + n.zm;
-                s = div_by_count(s, r_, d_, code);
-                s = s - n.c;
+                float qt = s * rdv.x;
+                const float e = __builtin_fmaf(-qt, rdv.y, s);
+                qt = __builtin_fmaf(e, rdv.x, qt);
+                // below 2^-100 the residual correction can be off (d = 6):
+                // IEEE division there (s = +-0 needs none: any zero quotient
+                // gives the same update)
+                if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
+                    qt = s / (code ? (float)code : __builtin_inff());
+                s = qt - n.c;
                 s = s * w;
                 out = n.c + s;
             } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
-                const int l = CP[o];
-                const float* row = sT + l * ncls;
-                const int lzm = act ? CP[oo] : CP[on];
-                const int lzp = act ? CP[on] : CP[oo];
-                out = update_multiphase(n, row[CPp[o]], row[CPm[o]], row[CP[o + kRowLen]],
-                                        row[CP[o - kRowLen]], row[lzp], row[lzm], p == 1, p == v.X,
-                                        w);
+                const float* row = sT + (int)Co[q][o] * ncls;
+                out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]], row[Co[q][o + kRowLen]],
+                                        row[Co[q][o - kRowLen]], row[Cp[q][o]], row[Cm[q][o]],
+                                        p == 1, p == v.X, w);
             } else {
-                const float kc = CP[o];
-                const float kzm = act ? CP[oo] : CP[on];
-                const float kzp = act ? CP[on] : CP[oo];
-                out = update_multiphase(n, hmean(kc, CPp[o]), hmean(kc, CPm[o]),
-                                        hmean(kc, CP[o + kRowLen]), hmean(kc, CP[o - kRowLen]),
-                                        hmean(kc, kzp), hmean(kc, kzm), p == 1, p == v.X, w);
+                const float kc = Co[q][o];
+                out = update_multiphase(n, hmean(kc, Co[q][op]), hmean(kc, Co[q][om]),
+                                        hmean(kc, Co[q][o + kRowLen]), hmean(kc, Co[q][o - kRowLen]),
+                                        hmean(kc, Cp[q][o]), hmean(kc, Cm[q][o]), p == 1, p == v.X,
+                                        w);
             }
-            if (ok) P[o] = out;
+            Fq[o] = out;
         }
     };
 
-    auto store_plane = [&](int q, int sq) {
-        const bool qok = q >= x0 && q < x1;
-        const float* P = sF + sq * PS;
-        const int bq = q * plane_f + zl * 4;
-        const int o0 = (qok && own0) ? bq : kOff, o1 = (qok && own1) ? bq + 4 : kOff;
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            const int lo = (wave + j * NW) * kRowLen + lane;
-            st_f32(P[lo], rd[j], o0);
-            st_f32(P[lo + kRowPairs], rd[j], o1);
-        }
-    };
-
-    const int S0 = max(x0 - (H - 1), 1);
-    const int S1 = (x1 - 1) + (H - 1);
-    auto wrap = [](int sl) { return sl >= NP ? sl - NP : (sl < 0 ? sl + NP : sl); };
-    using I0 = std::integral_constant<int, 0>;
-    // planes S0-1 .. S0+1 go straight to LDS; S0+2 .. S0+kPrefetch+1 are put in flight
-    const int b0 = (S0 - 1) % NP;  // LDS slot of plane S0-1
-    issue_load(S0 - 1, I0{});
-    commit_load(b0, I0{});
-    issue_load(S0, I0{});
-    commit_load(wrap(b0 + 1), I0{});
-    issue_load(S0 + 1, I0{});
-    commit_load(wrap(b0 + 2), I0{});
-    static_for<2, kPrefetch + 2>([&](auto d_c) {
+    // planes S0-1 .. S0+1 go straight to LDS (slots 0..2); S0+2 .. S0+NP are put in flight
+    issue_load(S0 - 1, ic<0>{});
+    commit_load(ic<0>{}, ic<0>{});
+    issue_load(S0, ic<1>{});
+    commit_load(ic<1>{}, ic<1>{});
+    issue_load(S0 + 1, ic<2>{});
+    commit_load(ic<2>{}, ic<2>{});
+    static_for<2, NP + 1>([&](auto d_c) {
         constexpr int d = decltype(d_c)::value;
-        issue_load(S0 + d, std::integral_constant<int, d % kPrefetch>{});
+        issue_load(S0 + d, ic<(d + 1) % NP>{});
     });
     __syncthreads();
 
-    // step s: put plane s+1+kPrefetch in flight, run half-sweep h on plane
+    // step s = s0 + u: put plane s+1+NP in flight, run half-sweep h on plane
     // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS.
-    // Steps past S1 (to complete the last group of kPrefetch) do no work
-    // but keep the memory instructions unconditional.  bs = LDS slot of plane s.
-    int bs0 = wrap(b0 + 1);
-    for (int s0 = S0; s0 <= S1; s0 += kPrefetch) {
-        static_for<0, kPrefetch>([&](auto u_c) {
+    // Steps past S1 (to complete the last iteration) do no work but keep the
+    // memory instructions unconditional.
+    for (int s0 = S0; s0 <= S1; s0 += NP) {
+        set_bases(A);
+        static_for<0, NP>([&](auto u_c) {
             constexpr int u = decltype(u_c)::value;
             const int s = s0 + u;
-            const int bs = wrap(bs0 + u);
-            issue_load(s + 1 + kPrefetch, std::integral_constant<int, (u + 1) % kPrefetch>{});
-#pragma unroll
-            for (int h = 0; h < H; ++h) {
-                const int sp = wrap(bs - h);
-                stage(h, s - h, sp, wrap(sp - 1), wrap(sp + 1));
-                if (h + 1 < H) __syncthreads();
-            }
-            store_plane(s - (H - 1), wrap(bs - (H - 1)));
-            commit_load(wrap(bs + 2), std::integral_constant<int, (u + 2) % kPrefetch>{});
+            issue_load(s + 1 + NP, ic<(u + 2) % NP>{});
+            static_for<0, H>([&](auto h_c) {
+                constexpr int h = decltype(h_c)::value;
+                stage(h_c, ic<(u & 1)>{}, ic<((u + 1 - h) % NP + NP) % NP>{}, s - h);
+                if constexpr (h + 1 < H) __syncthreads();
+            });
+            store_plane(s - (H - 1), ic<((u + 2 - H) % NP + NP) % NP>{});
+            commit_load(ic<(u + 3) % NP>{}, ic<(u + 3) % NP>{});
             __syncthreads();
         });
-        bs0 = wrap(bs0 + kPrefetch);
+        A ^= NP & 1;
     }
 }
 
@@ -720,8 +855,10 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const int tx = t % a.nTx;
     const int b = t / a.nTx;
     const size_t off = (size_t)b * a.v.sB;
+    const size_t coff = KIND == TF_KIND_FIELD ? off
+                                              : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                 static_cast<const code_t<KIND>*>(a.codes) + off, a.hm, a.ncls, a.v,
+                                 static_cast<const code_t<KIND>*>(a.codes) + coff, a.hm, a.ncls, a.v,
                                  a.w, a.colour0, a.TX, tz, ty, tx);
 }
 
@@ -729,8 +866,8 @@ template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
     using Sh = SweepShape<H, NW, RPW>;
     const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
-    size_t bytes = (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
-    bytes += kind == TF_KIND_MULTIPHASE ? (size_t)ncls * ncls * sizeof(float) : 16 * sizeof(float);
+    size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
+    bytes += kind == TF_KIND_MULTIPHASE ? (size_t)ncls * ncls * sizeof(float) : 18 * sizeof(float);
     return bytes;
 }
 
@@ -769,10 +906,11 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
 template <int H, int KIND>
 static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
     if (nw == 8 && rpw == 2) return launch_sweep<H, KIND, 8, 2>(a, B, st);
-    if (nw == 8 && rpw == 3) return launch_sweep<H, KIND, 8, 3>(a, B, st);
-    if (nw == 16 && rpw == 1) return launch_sweep<H, KIND, 16, 1>(a, B, st);
     if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled (use 8x2, 8x3, 16x1, 4x4)",
+    if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
+    if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
+    if (nw == 16 && rpw == 2) return launch_sweep<H, KIND, 16, 2>(a, B, st);
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled (use 8x2, 4x4, 12x2, 8x4, 16x2)",
                 nw, rpw);
 }
 
@@ -789,12 +927,28 @@ static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream
     }
 }
 
-static void default_tile(int H, int kind, tf_tile* t) {
-    t->waves = 8;
-    // 24 rows when deeper blocking needs the taller halo, unless the 4-byte
-    // per-voxel diffusivity of TF_KIND_FIELD leaves too little LDS for them
-    t->rows_per_wave = (H <= 3 || kind == TF_KIND_FIELD) ? 2 : 3;
-    t->tx = 64;
+// owned voxels of z per tile (SweepShape::TZ)
+static int tile_tz(int H) { return ((kRowLen - 4 * ((H + 3) / 4) - H) / 4) * 4; }
+
+// Default workgroup shape and x-tile length.  32-row tiles (16 waves x 2)
+// halve the y-halo overhead of 16-row tiles where their LDS ring fits
+// (one-byte codes, n_half <= 4) and the volume is tall enough; the x tile
+// is shortened for small volumes so that the grid still has about two
+// workgroups per CU.  Any shape gives identical results.
+static void default_tile(int H, int kind, int B, int X, int Y, int Z, tf_tile* t) {
+    const bool tall = kind != TF_KIND_FIELD && H <= 4 && Y >= 64;
+    t->waves = tall ? 16 : 8;
+    t->rows_per_wave = 2;
+    const long long ty = (long long)t->waves * t->rows_per_wave - 2 * H;
+    const long long tz = tile_tz(H);
+    const long long yz = (long long)B * ((Y + ty - 1) / ty) * ((Z + tz - 1) / tz);
+    const long long want = 512;  // workgroups
+    long long tx = 64;
+    if (yz * ((X + 63) / 64) < want) {
+        const long long nx = (want + yz - 1) / yz;
+        tx = std::max(8LL, (X + nx - 1) / nx);
+    }
+    t->tx = (int)std::min(tx, 64LL);
 }
 
 extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const float* hm_lut,
@@ -811,14 +965,15 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
         return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
     tf_tile t;
-    default_tile(n_half, kind, &t);
+    default_tile(n_half, kind, B, X, Y, Z, &t);
     if (tile) {
         if (tile->waves > 0) t.waves = tile->waves;
         if (tile->rows_per_wave > 0) t.rows_per_wave = tile->rows_per_wave;
         if (tile->tx > 0) t.tx = tile->tx;
     }
     if (t.tx < 1 || t.tx > 65536) return fail(-1, "invalid tile tx=%d", t.tx);
-    if ((long long)(X + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31))
+    if ((long long)(X + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64 ||
+        sweep_code_volume(X, Y, Z) >= (1LL << 31))
         return fail(-1, "padded volume of %lld bytes: the sweep addresses at most 2 GiB per volume",
                     (long long)(X + 2) * (Y + 2) * (Z + 2) * 4);
     SweepArgs a;
@@ -1195,8 +1350,8 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
     const int tx = t / nTy;
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
-    sweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.offset, nullptr, 0, v, vol.omega,
-                                           a.colour0, a.TX, tz, ty, tx);
+    sweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, 0, v,
+                                           vol.omega, a.colour0, a.TX, tz, ty, tx);
 }
 
 template <int H, int NW, int RPW>
@@ -1225,14 +1380,18 @@ static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
 template <int H>
 static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
     if (nw == 8 && rpw == 2) return launch_sweep_batch<H, 8, 2>(a, st);
-    if (nw == 8 && rpw == 3) return launch_sweep_batch<H, 8, 3>(a, st);
-    if (nw == 16 && rpw == 1) return launch_sweep_batch<H, 16, 1>(a, st);
     if (nw == 4 && rpw == 4) return launch_sweep_batch<H, 4, 4>(a, st);
+    if (nw == 12 && rpw == 2) return launch_sweep_batch<H, 12, 2>(a, st);
+    if (nw == 8 && rpw == 4) return launch_sweep_batch<H, 8, 4>(a, st);
+    if (nw == 16 && rpw == 2) return launch_sweep_batch<H, 16, 2>(a, st);
     return fail(-1, "workgroup shape %d waves x %d rows is not compiled", nw, rpw);
 }
 
 static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {
-    default_tile(n_half, TF_KIND_SOLVER, t);
+    // batches: volumes of different sizes share one shape; 16-row tiles
+    t->waves = 8;
+    t->rows_per_wave = 2;
+    t->tx = 64;
     if (tile) {
         if (tile->waves > 0) t->waves = tile->waves;
         if (tile->rows_per_wave > 0) t->rows_per_wave = tile->rows_per_wave;
@@ -1252,7 +1411,7 @@ extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_til
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
     const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
-    const long long tz = kRowLen - 2 * n_half;
+    const long long tz = tile_tz(n_half);
     long long n = ((Z + tz - 1) / tz) * ((Y + ty - 1) / ty) * ((X + t.tx - 1) / t.tx);
     if (n >= (1LL << 28)) return fail(-1, "too many tiles");
     *count = (int)n;

@@ -35,7 +35,7 @@ __all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "M
            "PeriodicMultiPhaseSolver", "FieldSolver"]
 
 # half-sweeps fused per HBM pass unless a solver is told otherwise
-DEFAULT_HALF_SWEEPS_PER_PASS = 2
+DEFAULT_HALF_SWEEPS_PER_PASS = 4
 # reference check cadence (taufactor/taufactor.py:184) and tau tolerance (:148)
 CHECK_EVERY = 100
 TAU_TOL = 2e-3
@@ -77,6 +77,7 @@ class SORSolver(ABC):
         self._cur = 0
         self._launch_events = None  # list -> (start, end, n_launches) per _advance
         self._codes = None
+        self._sweep_codes = None
         self._hm_lut = None
         self._n_classes = 0
         self._spans_cache = {}
@@ -157,6 +158,13 @@ class SORSolver(ABC):
         vec = torch.linspace(self.top_bc + sh, self.bot_bc - sh, self.Nx, dtype=torch.float32)
         return vec.to(self.device)
 
+    def _pack_sweep_codes(self) -> None:
+        """Repack the codes into the aligned layout the sweep reads."""
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        st = self._lib.tf_pack_sweep_codes(_lib.ptr(self._codes), _lib.ptr(self._sweep_codes),
+                                           B, X, Y, Z, self._stream())
+        _lib.check(st, "tf_pack_sweep_codes")
+
     def _advance(self, n_half: int) -> None:
         """Run n_half reference iterations (half-sweeps) on the device."""
         lib = self._lib
@@ -165,7 +173,9 @@ class SORSolver(ABC):
         if self.tile is not None:
             tile = _lib.Tile(*self.tile)
         stream = self._stream()
-        codes = _lib.ptr(self._codes)
+        # the sweep reads the aligned sweep codes (the per-voxel diffusivity
+        # itself for the field kind)
+        codes = _lib.ptr(self._sweep_codes if self._sweep_codes is not None else self._codes)
         hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
         it = self.iter
         per = max(1, min(int(self.half_sweeps_per_pass), _lib.MAX_FUSED_HALF_SWEEPS))
@@ -437,7 +447,8 @@ class Solver(ThroughTransportSolver):
             np.ascontiguousarray(self.cpu_img != 0, dtype=np.uint8)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
-        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        self._sweep_codes = _lib.sweep_code_buffer(B, X, Y, Z, self.device)
+        self._fields = [_lib.slack_buffer(shape, torch.float32, self.device) for _ in range(2)]
         self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
         self._xinit_dev = self._xinit()
 
@@ -448,6 +459,7 @@ class Solver(ThroughTransportSolver):
                                        _lib.ptr(self._fields[1]), _lib.ptr(self._count),
                                        B, X, Y, Z, self._setup_flags, self._stream())
         _lib.check(st, "tf_setup_solver")
+        self._pack_sweep_codes()
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     @property
@@ -545,7 +557,8 @@ class MultiPhaseSolver(ThroughTransportSolver):
         self._cls_cond = torch.from_numpy((d_cls > 0).astype(np.uint8)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
-        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        self._sweep_codes = _lib.sweep_code_buffer(B, X, Y, Z, self.device)
+        self._fields = [_lib.slack_buffer(shape, torch.float32, self.device) for _ in range(2)]
         self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
         self._xinit_dev = self._xinit()
 
@@ -557,6 +570,7 @@ class MultiPhaseSolver(ThroughTransportSolver):
                                            _lib.ptr(self._fields[1]), _lib.ptr(self._count),
                                            B, X, Y, Z, self._setup_flags, self._stream())
         _lib.check(st, "tf_setup_multiphase")
+        self._pack_sweep_codes()
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     def _class_image(self, labels, cls_of_label) -> np.ndarray:
@@ -682,8 +696,8 @@ class FieldSolver(ThroughTransportSolver):
             self._input_dev = torch.from_numpy(
                 np.ascontiguousarray(self.cpu_img, dtype=np.float32)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
-        self._codes = torch.empty(shape, dtype=torch.float32, device=self.device)
-        self._fields = [torch.empty(shape, dtype=torch.float32, device=self.device) for _ in range(2)]
+        self._codes = _lib.slack_buffer(shape, torch.float32, self.device)
+        self._fields = [_lib.slack_buffer(shape, torch.float32, self.device) for _ in range(2)]
         self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
         self._xinit_dev = self._xinit()
 

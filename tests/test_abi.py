@@ -61,3 +61,36 @@ def test_spans_x_host_entry_point():
         shape = tuple(int(v) for v in rng.integers(1, 8, size=3))
         mm = np.ascontiguousarray(rng.random(shape) < rng.uniform(0.2, 0.7), dtype=np.uint8)
         assert lib.tf_spans_x(mm.ctypes.data, *shape) == int(spans_x(mm.astype(bool)))
+
+
+def test_batch_struct_mirrors_match_c_layout():
+    """The numpy mirrors of tf_volume / tf_volume_state have the field
+    offsets and sizes C gives the header's structs."""
+    c_int, c_ll, c_f, c_d = ctypes.c_int, ctypes.c_longlong, ctypes.c_float, ctypes.c_double
+
+    class Volume(ctypes.Structure):
+        _fields_ = [("offset", c_ll), ("code_offset", c_ll), ("X", c_int), ("Y", c_int),
+                    ("Z", c_int), ("omega", c_f), ("sum_offset", c_int), ("tile_first", c_int),
+                    ("spans", c_int), ("reserved", c_int), ("D_mean", c_d)]
+
+    class State(ctypes.Structure):
+        _fields_ = [("active", c_int), ("converged", c_int), ("iters", c_int), ("cur", c_int),
+                    ("tau", c_d), ("old_tau", c_d), ("D_rel", c_d), ("rel_err", c_d)]
+
+    for cs, dt in ((Volume, _lib.VOLUME_DTYPE), (State, _lib.STATE_DTYPE)):
+        assert ctypes.sizeof(cs) == dt.itemsize
+        for name, _ in cs._fields_:
+            assert getattr(cs, name).offset == dt.fields[name][1], name
+    text = open(os.path.join(ROOT, "include", "tauhip.h")).read()
+    body = re.search(r"typedef struct tf_volume \{(.*?)\} tf_volume;", text, re.S).group(1)
+    names = re.findall(r"\b(\w+);", re.sub(r"/\*.*?\*/", "", body, flags=re.S))
+    assert names == [n for n, _ in Volume._fields_ if n not in ("X", "Y")]  # "int X, Y, Z;"
+
+
+def test_sweep_code_layout_size():
+    lib = _lib.load()
+    for X, Y, Z in [(1, 1, 1), (5, 7, 9), (512, 512, 512), (3, 4, 11)]:
+        row = ((Z + 5 + 3) // 4) * 4
+        assert row % 4 == 0 and row >= Z + 5
+        assert lib.tf_sweep_codes_bytes(X, Y, Z) == (X + 2) * (Y + 2) * row
+    assert lib.tf_sweep_codes_bytes(0, 1, 1) < 0

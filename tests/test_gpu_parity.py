@@ -47,9 +47,15 @@ def test_solver_iterates_bit_identical(shape, H):
     assert np.array_equal(_field(s), o.field.numpy())
 
 
-@pytest.mark.parametrize("tile", [(8, 2, 1), (16, 1, 5), (4, 4, 9), (8, 3, 2)])
+def _lds_bytes(tile, H, code_bytes=1):
+    return 512 + (H + 3) * tile[0] * tile[1] * 128 * (4 + code_bytes) + 72
+
+
+@pytest.mark.parametrize("tile", [(8, 2, 1), (4, 4, 9), (12, 2, 5), (8, 4, 2), (16, 2, 3)])
 @pytest.mark.parametrize("H", [2, 3, 4, 6])
 def test_solver_iterates_with_small_tiles(tile, H):
+    if _lds_bytes(tile, H) > 160 * 1024:
+        pytest.skip("shape needs more than 160 KiB of LDS at this depth")
     img = _binary((19, 23, 300), seed=5)
     s = tau.Solver(img)
     s.half_sweeps_per_pass = H
@@ -95,10 +101,23 @@ def test_multiphase_label_zero_conductive_and_odd_values():
     img = _labels((14, 15, 16), n=3, seed=9)
     Ds = {0: 0.7, 1: 0.0, 2: 1.0 / 3.0}
     s = tau.MultiPhaseSolver(img, dict(Ds))
-    s.tile = (16, 1, 7)
+    s.tile = (16, 2, 7)
     s.solve(iter_limit=33, verbose=False)
     o = OracleMultiPhaseSolver(img, dict(Ds)).run(33)
     assert np.array_equal(_field(s), o.field.numpy())
+
+
+@pytest.mark.parametrize("shape", [(3, 4, 5), (2, 3, 12), (4, 2, 1)])
+def test_sweep_code_layout(shape):
+    """tf_pack_sweep_codes: byte z+3 of each row holds code z, the rest 0."""
+    img = _binary(shape, seed=4)
+    s = tau.Solver(np.stack([img, 1 - img]))
+    X, Y, Z = shape
+    row = ((Z + 5 + 3) // 4) * 4
+    got = s._sweep_codes.cpu().numpy().reshape(2, X + 2, Y + 2, row)
+    want = np.zeros_like(got)
+    want[..., 3:Z + 5] = s._codes.cpu().numpy()
+    assert np.array_equal(got, want)
 
 
 def test_inplace_half_sweep_kernel_matches_oracle():
