@@ -93,7 +93,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--half-sweeps", type=int, default=None)
-    ap.add_argument("--tile", type=str, default=None, help="ty,tz,tx")
+    ap.add_argument("--tile", type=str, default=None, help="waves,rows_per_wave,tx")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config", default="solver512",
@@ -131,7 +131,11 @@ def main(argv=None) -> int:
         s.half_sweeps_per_pass = args.half_sweeps
     if args.tile:
         s.tile = tuple(int(v) for v in args.tile.split(","))
-    H = s.half_sweeps_per_pass
+    t0 = time.perf_counter()
+    H, tile = s._launch_config()  # runs the autotuner unless both are given
+    log(f"launch configuration: {H} half-sweeps per launch, tile {tile or 'library default'} "
+        f"({'autotuned' if args.half_sweeps is None or args.tile is None else 'given'}, "
+        f"{time.perf_counter() - t0:.2f} s)")
     if 100 % H:
         log(f"note: {H} half-sweeps per pass does not divide 100; launches will be mixed")
 
@@ -184,7 +188,8 @@ def main(argv=None) -> int:
             "tau": float(s.tau[0]),
             "D_eff": float(s.D_eff[0]),
             "half_sweeps_per_pass": H,
-            "tile": list(s.tile) if s.tile else "default",
+            "tile": list(tile) if tile else "default",
+            "launch_config": "autotuned" if args.half_sweeps is None or args.tile is None else "given",
             "volume_sha256": workloads.sha256(img),
         },
         "roofline": {
@@ -204,7 +209,7 @@ def main(argv=None) -> int:
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if t.get("half_sweeps_per_pass") == H and t.get("tile") == (list(s.tile) if s.tile else "default"):
+        if t.get("half_sweeps_per_pass") == H and t.get("tile") == (list(tile) if tile else "default"):
             out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
             out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch, "
                                                  "profiles/k_sweep_solver_%d_traffic.md"

@@ -1,0 +1,132 @@
+"""Launch-configuration autotuner for the fused sweep.
+
+Every launch configuration (fused half-sweeps per HBM pass H, workgroup
+shape, x-tile length) gives identical iterates, so the choice is purely a
+speed question.  The tuner times a few candidates on the solver's own volume
+and keeps the one with the lowest time per reference iteration:
+
+* plain solvers (Solver, MultiPhaseSolver, FieldSolver): a few launches of
+  each candidate on scratch copies of the current field, so the solver's
+  state is untouched;
+* BatchSolver: a few passes of each candidate over the whole packed batch
+  (H must be even and divide 100 there), after which the batch is re-packed.
+
+Results are cached per (kind, batch, shape) for the life of the process.
+Volumes below MIN_VOXELS are not tuned: the defaults (H = 4, library tile)
+serve them and tuning would cost more than it saves.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+MIN_VOXELS = 1 << 23
+TIMED_LAUNCHES = 3
+
+# (H values, workgroup shapes, x-tile lengths), in order of preference: the
+# first candidate within TIE of the fastest wins, which keeps the choice
+# stable against timing noise
+SOLVER_SPACE = ((4, 3, 2), ((16, 2), (12, 2), (8, 2)), (128, 64))
+FIELD_SPACE = ((4, 3, 2), ((8, 2), (4, 4)), (128, 64))
+BATCH_SPACE = ((4, 2), ((8, 2), (4, 4), (12, 2), (16, 2)), (64, 32))
+TIE = 0.02
+
+_cache: dict = {}
+
+
+def _candidates(space, fixed_h, fixed_tile):
+    hs, shapes, txs = space
+    if fixed_h is not None:
+        hs = (int(fixed_h),)
+    tiles = [(w, r, tx) for (w, r) in shapes for tx in txs]
+    if fixed_tile is not None:
+        tiles = [tuple(fixed_tile)]
+    return [(h, t) for h in hs for t in tiles]
+
+
+def _time(launch, n):
+    """Mean ms of n calls of launch() (after one untimed call), or None if a
+    call reports an error (e.g. a shape that needs too much LDS)."""
+    if launch() != 0:
+        return None
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(n):
+        if launch() != 0:
+            return None
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / n
+
+
+def tune_solver(s, fixed_h=None, fixed_tile=None):
+    """Best (H, tile) for a plain solver `s` (SORSolver subclass)."""
+    B, X, Y, Z = s.batch_size, s.Nx, s.Ny, s.Nz
+    key = ("plain", s.kind, B, X, Y, Z, fixed_h, None if fixed_tile is None else tuple(fixed_tile))
+    if key in _cache:
+        return _cache[key]
+    lib = s._lib
+    space = FIELD_SPACE if s.kind == _lib.KIND_FIELD else SOLVER_SPACE
+    shape = tuple(s.field.shape)
+    src = _lib.slack_buffer(shape, torch.float32, s.device)
+    dst = _lib.slack_buffer(shape, torch.float32, s.device)
+    src.copy_(s.field)
+    dst.copy_(s.field)
+    codes = _lib.ptr(s._sweep_codes if s._sweep_codes is not None else s._codes)
+    hm = _lib.ptr(s._hm_lut) if s._hm_lut is not None else None
+    stream = s._stream()
+    timings = []
+    for H, tile in _candidates(space, fixed_h, fixed_tile):
+        t = _lib.Tile(*tile)
+        bufs = [src, dst]
+        state = {"i": 0}
+
+        def launch():
+            i = state["i"]
+            state["i"] += 1
+            return lib.tf_sweep(_lib.ptr(bufs[i & 1]), _lib.ptr(bufs[1 - (i & 1)]), codes, hm,
+                                s._n_classes, s.kind, B, X, Y, Z, s._w32, 0, H, t, stream)
+
+        ms = _time(launch, TIMED_LAUNCHES)
+        if ms is not None:
+            timings.append((ms / H, H, tile))
+    del src, dst
+    result = _pick(timings)
+    _cache[key] = result
+    return result
+
+
+def _pick(timings):
+    """First candidate (in preference order) within TIE of the fastest."""
+    if not timings:
+        return (None, None)
+    fastest = min(t for t, _, _ in timings)
+    for t, H, tile in timings:
+        if t <= fastest * (1 + TIE):
+            return (H, tile)
+
+
+def tune_batch(b, fixed_h=None, fixed_tile=None):
+    """Best (H, tile) for a BatchSolver `b`; re-packs the batch afterwards."""
+    key = ("batch", tuple(b.shapes), fixed_h, None if fixed_tile is None else tuple(fixed_tile))
+    if key in _cache:
+        return _cache[key]
+    timings = []
+    try:
+        for H, tile in _candidates(BATCH_SPACE, fixed_h, fixed_tile):
+            b._make_plan(H, tile)
+
+            def launch():
+                return b._pass_status(0, 0, H)
+
+            ms = _time(launch, TIMED_LAUNCHES)
+            if ms is not None:
+                timings.append((ms / H, H, tile))
+    finally:
+        b._plan = None
+        b.reset()
+    result = _pick(timings)
+    _cache[key] = result
+    return result

@@ -20,7 +20,7 @@ from timeit import default_timer as timer
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 from .solver import CHECK_EVERY, Solver
 
 
@@ -58,8 +58,11 @@ class BatchSolver:
         self.device = Solver._init_device(device)
         self._lib = _lib.load()
         self.D_0 = D_0
-        self.half_sweeps_per_pass = 4
+        # launch configuration: None lets the autotuner (large batches) or the
+        # defaults (H = 4, 16-row tiles) decide; H must be even and divide 100
+        self.half_sweeps_per_pass = None
         self.tile = None
+        self.autotune = True
         self.shapes = [tuple(int(d) for d in i.shape) for i in imgs]
         V = self.n_volumes = len(imgs)
         vols = np.zeros(V, dtype=_lib.VOLUME_DTYPE)
@@ -138,8 +141,8 @@ class BatchSolver:
             self._setup_all()
         self._plan = None
 
-    def _make_plan(self, H):
-        tile = None if self.tile is None else _lib.Tile(*self.tile)
+    def _make_plan(self, H, tile):
+        tile = None if tile is None else _lib.Tile(*tile)
         vols = self._vols_host.copy()
         tile_vol = []
         for v, (X, Y, Z) in enumerate(self.shapes):
@@ -160,11 +163,13 @@ class BatchSolver:
         self._plan = dict(H=H, tile=tile, n_tiles=len(tile_vol))
 
     def _pass(self, src_b, colour, h):
+        _lib.check(self._pass_status(src_b, colour, h), "tf_batch_sweep")
+
+    def _pass_status(self, src_b, colour, h):
         p = self._plan
-        st = self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes),
+        return self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes),
                                       _lib.ptr(self._vols_dev), _lib.ptr(self._tile_vol_dev), p["n_tiles"],
                                       _lib.ptr(self._state_dev), src_b, colour, h, p["tile"], self._stream())
-        _lib.check(st, "tf_batch_sweep")
 
     def _check(self, src_b, iter_limit, conv_crit):
         st = self._lib.tf_batch_layer_sums(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._codes),
@@ -187,20 +192,33 @@ class BatchSolver:
         self._check(src_b, iter_limit, conv_crit)
         return src_b
 
+    def _launch_config(self):
+        """(H, tile): explicit settings win, then the autotuner for batches of
+        at least autotune.MIN_VOXELS voxels, then H = 4 and the default tile."""
+        H, tile = self.half_sweeps_per_pass, self.tile
+        if H is not None and tile is not None:
+            return int(H), tuple(tile)
+        aH, atile = None, None
+        if self.autotune and sum(X * Y * Z for X, Y, Z in self.shapes) >= autotune.MIN_VOXELS:
+            aH, atile = autotune.tune_batch(self, H, tile)
+        H = int(H) if H is not None else (aH or 4)
+        tile = tuple(tile) if tile is not None else atile
+        return H, tile
+
     def _state(self):
         return self._state_dev.cpu().numpy().view(_lib.STATE_DTYPE)
 
     def solve(self, iter_limit=10000, verbose=False, conv_crit=1e-2, use_graph=True):
         """Iterate every volume until its own convergence (or iter_limit, a
         multiple of 100).  Returns tau (float32 [V])."""
-        H = int(self.half_sweeps_per_pass)
+        H, tile = self._launch_config()
         if H % 2 or CHECK_EVERY % H or H > _lib.MAX_FUSED_HALF_SWEEPS:
             raise ValueError("half_sweeps_per_pass must be even and divide 100")
         if iter_limit % CHECK_EVERY:
             raise ValueError("BatchSolver needs iter_limit to be a multiple of 100")
         with torch.no_grad(), torch.cuda.device(self.device):
             start = timer()
-            self._make_plan(H)
+            self._make_plan(H, tile)
             n_periods = iter_limit // CHECK_EVERY
             src_b = 0
             done = 0

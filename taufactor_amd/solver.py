@@ -29,7 +29,7 @@ from timeit import default_timer as timer
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, autotune
 
 __all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "MultiPhaseSolver",
            "PeriodicMultiPhaseSolver", "FieldSolver"]
@@ -70,8 +70,12 @@ class SORSolver(ABC):
         self._w32 = float(np.float32(omega))
 
         self._lib = _lib.load()
-        self.half_sweeps_per_pass = DEFAULT_HALF_SWEEPS_PER_PASS
-        self.tile = None  # (waves, rows_per_wave, tx) or None for the library default
+        # launch configuration: None lets the autotuner (large volumes) or the
+        # defaults decide; any choice gives identical iterates
+        self.half_sweeps_per_pass = None
+        self.tile = None  # (waves, rows_per_wave, tx) or None
+        self.autotune = True
+        self._auto_cfg = None
 
         self._fields = None  # two padded float32 buffers (ping-pong)
         self._cur = 0
@@ -165,20 +169,41 @@ class SORSolver(ABC):
                                            B, X, Y, Z, self._stream())
         _lib.check(st, "tf_pack_sweep_codes")
 
+    _tunable = True
+
+    def _launch_config(self):
+        """(H, tile) of the next launches: explicit settings win; otherwise the
+        autotuner's choice for volumes of at least autotune.MIN_VOXELS voxels
+        (made once, before the first iteration), else H = 4 and the library's
+        default tile."""
+        H, tile = self.half_sweeps_per_pass, self.tile
+        if H is not None and tile is not None:
+            return int(H), tuple(tile)
+        if self._auto_cfg is None:
+            big = self.batch_size * self.Nx * self.Ny * self.Nz >= autotune.MIN_VOXELS
+            if self.autotune and self._tunable and big and self.iter == 0:
+                self._auto_cfg = autotune.tune_solver(self, H, tile)
+            else:
+                self._auto_cfg = (None, None)
+        aH, atile = self._auto_cfg
+        H = int(H) if H is not None else (aH or DEFAULT_HALF_SWEEPS_PER_PASS)
+        tile = tuple(tile) if tile is not None else atile
+        return H, tile
+
     def _advance(self, n_half: int) -> None:
         """Run n_half reference iterations (half-sweeps) on the device."""
         lib = self._lib
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
-        tile = None
-        if self.tile is not None:
-            tile = _lib.Tile(*self.tile)
+        per, tile = self._launch_config()
+        per = max(1, min(per, _lib.MAX_FUSED_HALF_SWEEPS))
+        if tile is not None:
+            tile = _lib.Tile(*tile)
         stream = self._stream()
         # the sweep reads the aligned sweep codes (the per-voxel diffusivity
         # itself for the field kind)
         codes = _lib.ptr(self._sweep_codes if self._sweep_codes is not None else self._codes)
         hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
         it = self.iter
-        per = max(1, min(int(self.half_sweeps_per_pass), _lib.MAX_FUSED_HALF_SWEEPS))
         ev = self._launch_events
         if ev is not None:
             t0 = torch.cuda.Event(enable_timing=True)
@@ -740,6 +765,7 @@ class _PeriodicYZ:
     ghost wrap and one half-sweep."""
 
     _setup_flags = _lib.SETUP_PERIODIC_YZ
+    _tunable = False  # one half-sweep per ghost wrap: there is nothing to fuse
 
     def apply_boundary_conditions(self):
         st = self._lib.tf_wrap_ghosts(_lib.ptr(self.field), self.batch_size, self.Nx, self.Ny, self.Nz,
