@@ -44,6 +44,14 @@ extern "C" {
  * voxel diffusivities (x ghosts copy the adjacent voxel, y/z ghosts 0);
  * faces use the MultiPhase harmonic mean of the two voxels. */
 #define TF_KIND_FIELD 2
+/* voxel-spacing weights (AnisotropicSolver): y and z neighbours weighted by
+ * Ky = (dx/dy)^2 and Kz = (dx/dz)^2.  `codes` as TF_KIND_SOLVER but set up
+ * with TF_SETUP_ANISOTROPIC (code = 1 + 48*xm + 16*xp + 8*ym + 4*yp + 2*zm + zp
+ * from the neighbour states, x states 0/1/2, 0 = inactive); `hm_lut` holds
+ * TF_ANISO_TABLE floats: the prefactor nn of codes 0..144 (inf for 0 and for
+ * a zero sum), then Ky and Kz rounded to float32. */
+#define TF_KIND_ANISOTROPIC 3
+#define TF_ANISO_TABLE 147
 
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by TZ voxels in z
@@ -72,6 +80,8 @@ const char* tf_last_error(void);
 
 /* set-up flags: y and z periodic (PeriodicSolver / PeriodicMultiPhaseSolver) */
 #define TF_SETUP_PERIODIC_YZ 1
+/* set-up flag of tf_setup_solver: codes of TF_KIND_ANISOTROPIC */
+#define TF_SETUP_ANISOTROPIC 2
 
 /* Set-up of a two-phase volume.  Replaces SORSolver.__init__ field/prefactor
  * construction: ThroughTransportSolver.init_field (taufactor/taufactor.py:282-291),
@@ -86,7 +96,10 @@ const char* tf_last_error(void);
  *           conductive voxels per x layer.
  *   flags   TF_SETUP_PERIODIC_YZ: neighbour counts wrap round in y and z
  *           (PeriodicSolver.init_conductive_neighbours,
- *           taufactor/taufactor.py:493-499)                               */
+ *           taufactor/taufactor.py:493-499)
+ *           TF_SETUP_ANISOTROPIC: codes of TF_KIND_ANISOTROPIC, the state of
+ *           the six neighbours (AnisotropicSolver.init_conductive_neighbours,
+ *           taufactor/taufactor.py:459-471); not combined with periodic */
 int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t* codes,
                     float* field_a, float* field_b, uint32_t* layer_count,
                     int B, int X, int Y, int Z, int flags, void* stream);

@@ -33,8 +33,11 @@ def _hm(a, b):
     return out
 
 
-def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 40000):
+def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 40000,
+                      face_scale=(1.0, 1.0, 1.0)):
     """k: float64 [X,Y,Z] voxel diffusivity (0 = non-conductive).
+    face_scale: weights of the x, y and z faces (AnisotropicSolver's 1, Ky,
+    Kz; the x boundary conductance 2k is not scaled).
     Returns (c [X,Y,Z] float64, flux_1d [X-1] float64, D_rel float)."""
     k = np.asarray(k, dtype=np.float64)
     X, Y, Z = k.shape
@@ -60,7 +63,7 @@ def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 4000
         sl_b[axis] = slice(1, None)
         ka, kb = k[tuple(sl_a)], k[tuple(sl_b)]
         ia, ib = idx[tuple(sl_a)], idx[tuple(sl_b)]
-        g = _hm(ka, kb)
+        g = _hm(ka, kb) * face_scale[axis]
         m = (ia >= 0) & (ib >= 0) & (g > 0)
         ga, a_, b_ = g[m], ia[m], ib[m]
         rows += [a_, b_]
@@ -97,6 +100,13 @@ def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 4000
 def solver_D_rel(img: np.ndarray, **kw) -> float:
     """D_rel of a binary image (conductive = 1)."""
     return solve_fixed_point((np.asarray(img) == 1).astype(np.float64), **kw)[2]
+
+
+def anisotropic_D_rel(img: np.ndarray, spacing, **kw) -> float:
+    """D_rel of a binary image with AnisotropicSolver's face weights."""
+    dx, dy, dz = spacing
+    return solve_fixed_point((np.asarray(img) == 1).astype(np.float64),
+                             face_scale=(1.0, (dx / dy) ** 2, (dx / dz) ** 2), **kw)[2]
 
 
 def multiphase_D_rel(img: np.ndarray, diffusivities: dict, **kw) -> float:

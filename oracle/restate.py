@@ -7,7 +7,8 @@ the HIP path, never part of it.
 It restates, op for op in float32 torch on the CPU, what the reference does:
   set-up      taufactor/taufactor.py:24-67 (SORSolver.__init__), 218-253
               (_init_chequerboard, _pad, _crop, _sum_by_rolling), 282-291
-              (init_field), 380-410 (Solver), 535-604 (MultiPhaseSolver)
+              (init_field), 380-410 (Solver), 422-478 (AnisotropicSolver),
+              535-604 (MultiPhaseSolver)
   iteration   taufactor/taufactor.py:95-103, 174-182, 606-613
   check       taufactor/taufactor.py:109-153 (check_convergence),
               293-331 (compute_metrics), 412-419 / 615-620 (vertical_flux)
@@ -223,6 +224,43 @@ class OracleSolver(OracleSolve):
         vf[self.factor[:, 0:-1] > 8] = 0
         vf[self.factor[:, 1:] > 8] = 0
         return vf
+
+
+class OracleAnisotropicSolver(OracleSolver):
+    """taufactor/taufactor.py:422-478: y and z neighbours weighted by
+    Ky = (dx/dy)^2 and Kz = (dx/dz)^2 (Python floats, as the reference holds
+    them; torch rounds them to float32 in each product)."""
+
+    def __init__(self, img, spacing, omega=None, D_0=1, reduce="f64"):
+        dx, dy, dz = spacing
+        self.Ky = (dx / dy) ** 2
+        self.Kz = (dx / dz) ** 2
+        super().__init__(img, omega, D_0, reduce)
+
+    def init_factor(self, torch_img, mask):
+        """AnisotropicSolver.init_conductive_neighbours (taufactor/taufactor.py:459-471):
+        rolls by +1 then -1 along x, y, z of the padded mask, each times its
+        weight, accumulated in that order; the rolled value at an interior
+        voxel is its neighbour at -1 / +1."""
+        p = pad(mask, [2, 2])
+        inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        nn = torch.zeros_like(mask)
+        weight = [1.0, self.Ky, self.Kz]
+        for dim in range(1, 4):
+            for dr in (1, -1):
+                sl = list(inner)
+                sl[dim] = slice(0, -2) if dr == 1 else slice(2, None)
+                nn += p[tuple(sl)] * weight[dim - 1]
+        nn[mask == 0] = torch.inf
+        nn[nn == 0] = torch.inf
+        return nn
+
+    def weighted_sum(self):
+        """AnisotropicSolver.sum_weighted_neighbours (taufactor/taufactor.py:473-478)"""
+        f = self.field
+        return f[:, 2:, 1:-1, 1:-1] + f[:, :-2, 1:-1, 1:-1] + \
+            self.Ky * (f[:, 1:-1, 2:, 1:-1] + f[:, 1:-1, :-2, 1:-1]) + \
+            self.Kz * (f[:, 1:-1, 1:-1, 2:] + f[:, 1:-1, 1:-1, :-2])
 
 
 def harmonic_mean(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:

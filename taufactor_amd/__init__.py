@@ -7,9 +7,9 @@ buffers.  There is no CPU code path.
 """
 from .batch import BatchSolver
 from .autograd import effective_diffusivity
-from .solver import (FieldSolver, MultiPhaseSolver, PeriodicMultiPhaseSolver, PeriodicSolver, SORSolver,
-                     Solver, ThroughTransportSolver)
+from .solver import (AnisotropicSolver, FieldSolver, MultiPhaseSolver, PeriodicMultiPhaseSolver,
+                     PeriodicSolver, SORSolver, Solver, ThroughTransportSolver)
 
-__all__ = ["Solver", "MultiPhaseSolver", "PeriodicSolver", "PeriodicMultiPhaseSolver", "BatchSolver",
-           "FieldSolver", "effective_diffusivity", "SORSolver", "ThroughTransportSolver"]
+__all__ = ["Solver", "MultiPhaseSolver", "PeriodicSolver", "PeriodicMultiPhaseSolver", "AnisotropicSolver",
+           "BatchSolver", "FieldSolver", "effective_diffusivity", "SORSolver", "ThroughTransportSolver"]
 __version__ = "0.1.0"

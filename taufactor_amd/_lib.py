@@ -16,7 +16,10 @@ ABI_VERSION = 3
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
 KIND_FIELD = 2
+KIND_ANISOTROPIC = 3
+ANISO_TABLE = 147
 SETUP_PERIODIC_YZ = 1
+SETUP_ANISOTROPIC = 2
 MAX_FUSED_HALF_SWEEPS = 6
 # readable bytes the sweep needs before and after each field buffer (TF_FIELD_SLACK_BYTES)
 FIELD_SLACK_BYTES = 16

@@ -140,7 +140,7 @@ template <int MODE>
 __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
                         const float* __restrict__ xinit, uint8_t* __restrict__ codes,
                         float* __restrict__ fa, float* __restrict__ fb, Vol v, int B,
-                        int periodic) {
+                        int periodic, int aniso) {
     long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long total = (long long)B * v.sB;
     if (gid >= total) return;
@@ -164,7 +164,20 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
         int cond = MODE == 0 ? (m != 0) : (int)cls_cond[m];
         // mask * linspace in float32 (taufactor/taufactor.py:291)
         f = (cond ? 1.0f : 0.0f) * xinit[x];
-        if (MODE == 0) {
+        if (MODE == 0 && aniso) {
+            if (cond) {
+                // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
+                // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
+                // order (taufactor/taufactor.py:459-471)
+                const int xm = (x >= 1) ? (m_at(x - 1, y, z) != 0) : 2;
+                const int xq = (x + 1 < v.X) ? (m_at(x + 1, y, z) != 0) : 2;
+                const int ym = (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
+                const int yq = (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
+                const int zm = (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
+                const int zq = (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
+                code = (uint8_t)(1 + 48 * xm + 16 * xq + 8 * ym + 4 * yq + 2 * zm + zq);
+            }
+        } else if (MODE == 0) {
             if (cond) {
                 // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410),
                 // or y/z wrapped round (PeriodicSolver, taufactor/taufactor.py:493-499)
@@ -240,7 +253,10 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
                         uint8_t* codes, float* fa, float* fb, uint32_t* cnt,
                         int B, int X, int Y, int Z, int flags, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
-    if (flags & ~TF_SETUP_PERIODIC_YZ) return fail(-1, "unknown set-up flags 0x%x", flags);
+    if (flags & ~(TF_SETUP_PERIODIC_YZ | TF_SETUP_ANISOTROPIC))
+        return fail(-1, "unknown set-up flags 0x%x", flags);
+    if ((flags & TF_SETUP_ANISOTROPIC) && (MODE != 0 || (flags & TF_SETUP_PERIODIC_YZ)))
+        return fail(-1, "TF_SETUP_ANISOTROPIC applies to tf_setup_solver without periodic boundaries");
     if (!mask || !xinit || !codes || !fa || !fb || !cnt || (MODE == 1 && !cls_cond))
         return fail(-1, "null pointer argument");
     if (fa == fb) return fail(-1, "field_a and field_b must be distinct buffers");
@@ -251,7 +267,7 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
     long long blocks = (total + threads - 1) / threads;
     hipLaunchKernelGGL(k_setup<MODE>, dim3((unsigned)blocks), dim3(threads), 0, st,
                        mask, cls_cond, xinit, codes, fa, fb, v, B,
-                       (flags & TF_SETUP_PERIODIC_YZ) ? 1 : 0);
+                       (flags & TF_SETUP_PERIODIC_YZ) ? 1 : 0, (flags & TF_SETUP_ANISOTROPIC) ? 1 : 0);
     TF_LAUNCH_CHECK("k_setup");
     hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, B), dim3(256), 0, st, mask, cls_cond, cnt, v);
     TF_LAUNCH_CHECK("k_layer_count");
@@ -588,7 +604,12 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
         if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
     } else if (KIND == TF_KIND_MULTIPHASE) {
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
+    } else if (KIND == TF_KIND_ANISOTROPIC) {
+        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
     }
+    // spacing weights of TF_KIND_ANISOTROPIC (wave-uniform)
+    const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
+    const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
 
     // ---- global memory ----
     // Loads: one descriptor per array for the whole tile.  The field (and
@@ -791,6 +812,20 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
                 s = qt - n.c;
                 s = s * w;
                 out = n.c + s;
+            } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
+                // ((x+ + x-) + Ky*(y+ + y-)) + Kz*(z+ + z-), / nn
+                // (AnisotropicSolver.sum_weighted_neighbours, taufactor/taufactor.py:473-478)
+                float s = n.xp + n.xm;
+                float t = n.yp + n.ym;
+                t = ky * t;
+                s = s + t;
+                t = n.zp + n.zm;
+                t = kz * t;
+                s = s + t;
+                s = s / sT[Co[q][o]];
+                s = s - n.c;
+                s = s * w;
+                out = n.c + s;
             } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
                 const float* row = sT + (int)Co[q][o] * ncls;
                 out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]], row[Co[q][o + kRowLen]],
@@ -867,7 +902,9 @@ static size_t sweep_lds_bytes(int kind, int ncls) {
     using Sh = SweepShape<H, NW, RPW>;
     const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
     size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
-    bytes += kind == TF_KIND_MULTIPHASE ? (size_t)ncls * ncls * sizeof(float) : 18 * sizeof(float);
+    bytes += kind == TF_KIND_MULTIPHASE    ? (size_t)ncls * ncls * sizeof(float)
+             : kind == TF_KIND_ANISOTROPIC ? TF_ANISO_TABLE * sizeof(float)
+                                           : 18 * sizeof(float);
     return bytes;
 }
 
@@ -903,15 +940,18 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
 }
 
 // workgroup shapes (waves x rows per wave) that are compiled
+// compiled workgroup shapes: all five for TF_KIND_SOLVER, 8x2 / 4x4 / 16x2
+// for the other kinds
 template <int H, int KIND>
 static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
     if (nw == 8 && rpw == 2) return launch_sweep<H, KIND, 8, 2>(a, B, st);
     if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
-    if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
-    if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
     if (nw == 16 && rpw == 2) return launch_sweep<H, KIND, 16, 2>(a, B, st);
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled (use 8x2, 4x4, 12x2, 8x4, 16x2)",
-                nw, rpw);
+    if constexpr (KIND == TF_KIND_SOLVER) {
+        if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
+        if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
+    }
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for kind %d", nw, rpw, KIND);
 }
 
 template <int KIND>
@@ -957,10 +997,13 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     if (int rc = check_dims(B, X, Y, Z)) return rc;
     if (!src || !dst || !codes) return fail(-1, "null pointer argument");
     if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
-    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD)
+    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD &&
+        kind != TF_KIND_ANISOTROPIC)
         return fail(-1, "unknown kind %d", kind);
     if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 64))
         return fail(-1, "multiphase sweep needs hm_lut and 1..64 classes (got %d)", n_classes);
+    if (kind == TF_KIND_ANISOTROPIC && (!hm_lut || n_classes != TF_ANISO_TABLE))
+        return fail(-1, "anisotropic sweep needs the %d-entry prefactor table", TF_ANISO_TABLE);
     if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
         return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
@@ -987,6 +1030,8 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
         return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_MULTIPHASE)
         return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
+    if (kind == TF_KIND_ANISOTROPIC)
+        return dispatch_sweep<TF_KIND_ANISOTROPIC>(n_half, a, B, t.waves, t.rows_per_wave, st);
     return dispatch_sweep<TF_KIND_FIELD>(n_half, a, B, t.waves, t.rows_per_wave, st);
 }
 
@@ -1024,6 +1069,10 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
                 float d = cn - c;
                 if constexpr (KIND == TF_KIND_SOLVER) {
                     if (Cv[o] == 0 || Cv[o + v.sX] == 0) d = 0.0f;
+                } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
+                    // Solver.vertical_flux masks faces of voxels whose factor
+                    // exceeds 8 (taufactor/taufactor.py:417-418)
+                    if (hm[Cv[o]] > 8.0f || hm[Cv[o + v.sX]] > 8.0f) d = 0.0f;
                 } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
                     d = hm[Cv[o] * ncls + Cv[o + v.sX]] * d;
                 } else {
@@ -1087,6 +1136,9 @@ extern "C" int tf_layer_sums(const float* field, const void* codes, const float*
     else if (kind == TF_KIND_FIELD)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_FIELD>, grid, dim3(kSumThreads), 0, st, field,
                            static_cast<const float*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
+    else if (kind == TF_KIND_ANISOTROPIC && hm_lut && n_classes == TF_ANISO_TABLE)
+        hipLaunchKernelGGL(k_layer_sums<TF_KIND_ANISOTROPIC>, grid, dim3(kSumThreads), 0, st, field,
+                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
     else
         return fail(-1, "unknown kind %d", kind);
     TF_LAUNCH_CHECK("k_layer_sums");

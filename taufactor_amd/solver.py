@@ -31,8 +31,8 @@ import torch
 
 from . import _lib, autotune
 
-__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "MultiPhaseSolver",
-           "PeriodicMultiPhaseSolver", "FieldSolver"]
+__all__ = ["SORSolver", "ThroughTransportSolver", "Solver", "PeriodicSolver", "AnisotropicSolver",
+           "MultiPhaseSolver", "PeriodicMultiPhaseSolver", "FieldSolver"]
 
 # half-sweeps fused per HBM pass unless a solver is told otherwise
 DEFAULT_HALF_SWEEPS_PER_PASS = 4
@@ -501,6 +501,67 @@ class Solver(ThroughTransportSolver):
         vert_flux[fac[:, 0:-1] > 8] = 0
         vert_flux[fac[:, 1:] > 8] = 0
         return vert_flux
+
+
+class AnisotropicSolver(Solver):
+    """Two-phase solver with voxel-spacing weights (taufactor/taufactor.py:422-478).
+
+    y neighbours are weighted by Ky = (dx/dy)^2 and z neighbours by
+    Kz = (dx/dz)^2, in the prefactor and in the neighbour sum.
+
+    Args:
+        img (numpy.ndarray): Binary image with labels in ``{0, 1}``.
+        spacing (tuple[float, float, float]): voxel spacing ``(dx, dy, dz)``.
+        omega, D_0, device: as Solver.
+
+    Raises:
+        ValueError: If ``spacing`` is not three numbers.
+        UserWarning: If the spacings differ by more than a factor 10.
+    """
+
+    kind = _lib.KIND_ANISOTROPIC
+    _setup_flags = _lib.SETUP_ANISOTROPIC
+    def __init__(self, img, spacing, omega=None, D_0=1, device="cuda"):
+        if not isinstance(spacing, (list, tuple)) or len(spacing) != 3:
+            raise ValueError("spacing must be a list or tuple with three elements (dx, dy, dz)")
+        if not all(isinstance(x, (int, float)) for x in spacing):
+            raise ValueError("All elements in spacing must be integers or floats")
+        if np.max(spacing) / np.min(spacing) > 10:
+            warnings.warn("This computation is very questionable for largely different spacings e.g. dz >> dx.")
+        dx, dy, dz = spacing
+        self.Ky = (dx / dy) ** 2
+        self.Kz = (dx / dz) ** 2
+        super().__init__(img, omega=omega, D_0=D_0, device=device)
+
+    def _nn_table(self) -> np.ndarray:
+        """Prefactor of every neighbour state, summed in float32 in the
+        reference's order: x-, x+, Ky*y-, Ky*y+, Kz*z-, Kz*z+ (the rolls of
+        AnisotropicSolver.init_conductive_neighbours, taufactor/taufactor.py:459-471),
+        inf for zero; then Ky and Kz as float32."""
+        one, ky, kz = np.float32(1.0), np.float32(self.Ky), np.float32(self.Kz)
+        table = np.full(_lib.ANISO_TABLE, np.inf, dtype=np.float32)
+        for code in range(1, 145):
+            i = code - 1
+            terms = (np.float32(i // 48) * one, np.float32((i // 16) % 3) * one,
+                     np.float32((i >> 3) & 1) * ky, np.float32((i >> 2) & 1) * ky,
+                     np.float32((i >> 1) & 1) * kz, np.float32(i & 1) * kz)
+            nn = np.float32(0.0)
+            for t in terms:
+                nn = np.float32(nn + t)
+            table[code] = nn if nn != 0 else np.inf
+        table[145], table[146] = ky, kz
+        return table
+
+    def _setup_device(self):
+        super()._setup_device()
+        self._hm_lut = torch.from_numpy(self._nn_table()).to(self.device)
+        self._n_classes = _lib.ANISO_TABLE
+
+    @property
+    def factor(self) -> torch.Tensor:
+        """Weighted neighbour sum, inf where the update is void (taufactor/taufactor.py:459-471)."""
+        c = self._codes[:, 1:-1, 1:-1, 1:-1].long()
+        return self._hm_lut[:145][c]
 
 
 class MultiPhaseSolver(ThroughTransportSolver):
