@@ -177,3 +177,27 @@ def test_multiphase_full_solve_matches_oracle():
     assert s.iter == o.iter
     assert np.array_equal(_field(s), o.field.numpy())
     np.testing.assert_allclose(s.tau, o.tau, rtol=2e-7)
+
+
+def test_integration_stub_runs():
+    """The ctypes binding shown to reference maintainers works as printed."""
+    import os
+    import re
+
+    from taufactor_amd import _lib
+
+    text = open(os.path.join(os.path.dirname(os.path.dirname(__file__)), "INTEGRATION.md")).read()
+    code = re.search(r"```python\nimport ctypes, math, torch\n(.*?)```", text, re.S).group(0)
+    code = code.strip("`").removeprefix("python\n").replace('"libtauhip.so"', repr(_lib.LIB_PATH))
+    ns = {}
+    exec(compile(code, "INTEGRATION.md", "exec"), ns)
+    img = _binary((11, 9, 13), seed=8)
+    w = 2 - np.pi / (1.5 * 11)
+    st = ns["HipSolverState"](torch.from_numpy(img[None]).cuda(), w)
+    st.iterate(0, 10)
+    st.iterate(10, 7)
+    o = OracleSolver(img).run(17)
+    assert np.array_equal(st.f[st.cur].cpu().numpy(), o.field.numpy())
+    flux, conc = st.layer_means()
+    vf = o.vertical_flux().to(torch.float64).mean((2, 3)).numpy()
+    np.testing.assert_allclose(flux, vf, rtol=1e-12, atol=1e-15)
