@@ -52,6 +52,13 @@ extern "C" {
  * a zero sum), then Ky and Kz rounded to float32. */
 #define TF_KIND_ANISOTROPIC 3
 #define TF_ANISO_TABLE 147
+/* transport-reaction electrode (ElectrodeSolver): `codes` from
+ * tf_setup_electrode (code = 1 + 7*conductive neighbours + reactive
+ * neighbours, 0 = not conductive); `hm_lut` holds TF_ELECTRODE_TABLE floats
+ * per batch element: the prefactor cond + k0*reac of each code (inf for 0 and
+ * for a zero sum). */
+#define TF_KIND_ELECTRODE 4
+#define TF_ELECTRODE_TABLE 64
 
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by TZ voxels in z
@@ -118,6 +125,27 @@ int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
                         float* field_a, float* field_b, uint32_t* layer_count,
                         int B, int X, int Y, int Z, int flags, void* stream);
 
+/* Set-up of a transport-reaction electrode volume.  Replaces
+ * ElectrodeSolver.return_mask / init_field / init_conductive_neighbours /
+ * init_reactive_neighbours (taufactor/electrode.py:36-70; the periodic forms
+ * :133-151) and the reactive-surface sum of SORSolver.__init__
+ * (taufactor/taufactor.py:47-48).
+ *   cls      device uint8 [B][X][Y][Z]: 1 conductive, 2 reactive, 0 inert
+ *   xinit    device float [X], the initial profile cosh(1 - (x+0.5)/X)/cosh(1)
+ *            rounded to float32
+ *   codes    device uint8 padded: 1 + 7*cond + reac for conductive voxels
+ *            (cond counts the x = 0 ghost twice and the x = X+1 ghost not at
+ *            all; reac counts reactive neighbours, ghosts excluded), 0 else
+ *   field_a, field_b  padded initial field; x ghosts 2 (x = 0) and 0 (x = X+1)
+ *   layer_count     device uint32 [B][X], conductive voxels per layer
+ *   reactive_count  device uint32 [B][X], sum of reac over the layer's
+ *            conductive voxels
+ *   flags    TF_SETUP_PERIODIC_YZ: neighbour counts wrap round in y and z */
+int tf_setup_electrode(const uint8_t* cls, const float* xinit, uint8_t* codes,
+                       float* field_a, float* field_b, uint32_t* layer_count,
+                       uint32_t* reactive_count, int B, int X, int Y, int Z, int flags,
+                       void* stream);
+
 /* Periodic boundaries: every y/z ghost of every x plane takes the value of
  * the opposite interior layer (corners both wraps).  Replaces
  * apply_boundary_conditions of PeriodicSolver / PeriodicMultiPhaseSolver
@@ -177,7 +205,11 @@ int tf_sweep_half_inplace(float* field, const uint8_t* codes,
  * reductions of ThroughTransportSolver.compute_metrics (:296, :307).
  *   flux_sum device double [B][X-1]: sum over (y,z) of the float32 face flux
  *            (c[x+1]-c[x], zeroed when either voxel has code 0, for
- *            TF_KIND_SOLVER; hm*(c[x+1]-c[x]) for TF_KIND_MULTIPHASE)
+ *            TF_KIND_SOLVER; hm*(c[x+1]-c[x]) for TF_KIND_MULTIPHASE;
+ *            zeroed where the factor exceeds 8 for TF_KIND_ANISOTROPIC).
+ *            TF_KIND_ELECTRODE: [B][X], ElectrodeSolver.compute_metrics'
+ *            fluxes (taufactor/electrode.py:80-83): c[x-1]-c[x] for x >= 1,
+ *            (1 - c[0])/0.5 for x = 0, zeroed where either value is 0
  *   conc_sum device double [B][X]:   sum over (y,z) of c
  * The summation order is fixed, so repeated calls give identical bits.    */
 int tf_layer_sums(const float* field, const void* codes,

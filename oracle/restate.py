@@ -8,7 +8,8 @@ It restates, op for op in float32 torch on the CPU, what the reference does:
   set-up      taufactor/taufactor.py:24-67 (SORSolver.__init__), 218-253
               (_init_chequerboard, _pad, _crop, _sum_by_rolling), 282-291
               (init_field), 380-410 (Solver), 422-478 (AnisotropicSolver),
-              535-604 (MultiPhaseSolver)
+              535-604 (MultiPhaseSolver); taufactor/electrode.py:13-157
+              (ElectrodeSolver, PeriodicElectrodeSolver)
   iteration   taufactor/taufactor.py:95-103, 174-182, 606-613
   check       taufactor/taufactor.py:109-153 (check_convergence),
               293-331 (compute_metrics), 412-419 / 615-620 (vertical_flux)
@@ -414,3 +415,137 @@ class OraclePeriodicMultiPhaseSolver(OracleMultiPhaseSolver):
     def step(self):
         _wrap_yz(self.field)
         super().step()
+
+
+class OracleElectrodeSolver(OracleSolve):
+    """taufactor/electrode.py:13-129 (ElectrodeSolver) with the reaction
+    prefactor of SORSolver.__init__ (taufactor/taufactor.py:24-67): factor =
+    cond_nn + k0 * reac_nn, left x ghost 2 with weight 2, right x face closed,
+    tau from the transmission-line impedance of the layer fluxes."""
+
+    periodic = False
+
+    def __init__(self, img, conductive_label=1, reactive_label=0, omega=None, spacing=None, reduce="f64"):
+        self.left_bc, self.electrode_bc = 1.0, 0.0
+        self.cond_label, self.reac_label = conductive_label, reactive_label
+        self.dx = spacing or 1
+        self.cpu_img = expand_to_4d(np.asarray(img))
+        self.B, self.Nx, self.Ny, self.Nz = self.cpu_img.shape
+        self.omega = default_omega(self.Nx) if omega is None else omega
+        self.reduce = reduce
+        torch_img = torch.tensor(self.cpu_img, dtype=torch.float32)
+        mask = torch.zeros_like(torch_img)
+        mask[torch_img == self.cond_label] = 1
+        vol_x = torch.mean(mask, (2, 3))
+        self.field = self._init_field(mask)
+        self.factor = self._cond_nn(mask)
+        reac_nn = self._reac_nn(torch_img)
+        a_x = torch.sum(reac_nn, (2, 3)) / (self.Ny * self.Nz * self.dx)
+        k_0 = torch.mean(vol_x, 1) / torch.mean(a_x * self.dx, 1) / self.Nx ** 2
+        reac_nn = reac_nn * k_0[:, None, None, None]
+        self.factor += reac_nn
+        self.factor[self.factor == 0] = torch.inf
+        self.a_x, self.k_0 = a_x.numpy(), k_0.numpy()
+        self.vol_x = vol_x.numpy()
+        self.cb = chequerboards(self.Nx, self.Ny, self.Nz, self.omega)
+        self.iter = 0
+        self.old_tau = 0
+        self.converged = False
+        self.tau = None
+        self.tau_x = None
+        self.c_x = 0
+        self.n_checks = 0
+
+    def _init_field(self, mask):
+        """taufactor/electrode.py:39-46"""
+        x = np.arange(self.Nx) + 0.5
+        c = self.electrode_bc + (self.left_bc - self.electrode_bc) * np.cosh(1 - x / self.Nx) / np.cosh(1)
+        vec = torch.tensor(c, dtype=torch.float32)[None, :, None, None].repeat(self.B, 1, self.Ny, self.Nz)
+        return pad(mask * vec, [self.left_bc * 2, 0])
+
+    @staticmethod
+    def _roll_sum(t):
+        s = torch.zeros_like(t)
+        for dim in range(1, 4):
+            for dr in (1, -1):
+                s += torch.roll(t, dr, dim)
+        return s
+
+    def _cond_nn(self, mask):
+        """taufactor/electrode.py:48-54 (periodic: :133-142)"""
+        if self.periodic:
+            nn = self._roll_sum(pad(mask, [2, 0])[:, :, 1:-1, 1:-1])[:, 1:-1]
+        else:
+            nn = self._roll_sum(pad(mask, [2, 0]))[:, 1:-1, 1:-1, 1:-1]
+        nn[mask == 0] = torch.inf
+        return nn
+
+    def _reac_nn(self, img):
+        """taufactor/electrode.py:56-62 (periodic: :144-151)"""
+        img2 = torch.zeros_like(img)
+        img2[img == self.reac_label] = 1
+        if self.periodic:
+            nn = self._roll_sum(pad(img2, [0, 0])[:, :, 1:-1, 1:-1])[:, 1:-1]
+        else:
+            nn = self._roll_sum(pad(img2, [0, 0]))[:, 1:-1, 1:-1, 1:-1]
+        nn[img != self.cond_label] = 0.0
+        return nn
+
+    def weighted_sum(self):
+        """SORSolver.sum_weighted_neighbours (taufactor/taufactor.py:95-103)"""
+        f = self.field
+        return (f[:, 2:, 1:-1, 1:-1] + f[:, :-2, 1:-1, 1:-1] + f[:, 1:-1, 2:, 1:-1] +
+                f[:, 1:-1, :-2, 1:-1] + f[:, 1:-1, 1:-1, 2:] + f[:, 1:-1, 1:-1, :-2])
+
+    def step(self):
+        if self.periodic:
+            _wrap_yz(self.field)
+        super().step()
+
+    def compute_metrics(self):
+        """taufactor/electrode.py:72-104"""
+        f = self.field
+        inner = f[:, 1:-1, 1:-1, 1:-1]
+        vol_x = self.vol_x if self.reduce == "f32" else self.vol_x.astype(np.float64)
+        c_x = self._layer_mean(inner)
+        c_x = np.divide(c_x, vol_x, out=np.zeros_like(vol_x), where=vol_x != 0)
+        relative_error = np.max(np.abs(c_x - self.c_x), axis=1)
+        self.c_x = c_x
+        fl = -inner + f[:, :-2, 1:-1, 1:-1]
+        fl[:, 0, :, :] = (self.left_bc - f[:, 1, 1:-1, 1:-1]) / 0.5
+        fl[inner == 0] = 0
+        fl[f[:, :-2, 1:-1, 1:-1] == 0] = 0
+        fluxes = self._layer_mean(fl)
+        fluxes_1d = np.concatenate((2 * (self.left_bc - c_x[:, :1]), (-c_x[:, 1:] + c_x[:, :-1])), axis=1)
+        fluxes_1d[:, 1:][vol_x[:, 1:] == 0] = 0
+        fluxes_1d[:, 1:][vol_x[:, :-1] == 0] = 0
+        eps = np.concatenate((vol_x[:, :1], 0.5 * (vol_x[:, :-1] + vol_x[:, 1:])), axis=1)
+        self.tau_x = np.divide(eps * fluxes_1d, fluxes, out=np.full_like(fluxes_1d, np.nan),
+                               where=fluxes != 0)
+        fluxes[:, :-1] -= fluxes[:, 1:]
+        self.k_x = np.divide(fluxes, c_x - self.electrode_bc, out=np.zeros_like(c_x),
+                             where=(c_x - self.electrode_bc) != 0) / self.k_0[:, None]
+        a_dx = self.a_x * self.dx if self.reduce == "f32" else self.a_x.astype(np.float64) * self.dx
+        freq = np.mean(eps, axis=1, keepdims=True) / np.mean(a_dx, axis=1, keepdims=True) / self.Nx ** 2 * 2 ** -3
+        R = self.tau_x / eps
+        R[eps == 0] = 1e30
+        R[np.isnan(self.tau_x)] = 1e30
+        Z_sim = _impedance_batched(R, a_dx, freq)
+        R_ideal = 1 / np.mean(eps, axis=1)[:, None].repeat(self.Nx, axis=1)
+        C_ideal = np.mean(a_dx, axis=1)[:, None].repeat(self.Nx, axis=1)
+        Z_ideal = _impedance_batched(R_ideal, C_ideal, freq)
+        return Z_sim[:, 0].real / Z_ideal[:, 0].real, relative_error
+
+
+class OraclePeriodicElectrodeSolver(OracleElectrodeSolver):
+    """taufactor/electrode.py:131-157"""
+
+    periodic = True
+
+
+def _impedance_batched(R, C, w):
+    """compute_impedance_batched (taufactor/utils.py:87-98) for w of shape (bs, F)."""
+    Z = np.full_like(w, 1e50, dtype=complex)
+    for k in range(R.shape[1] - 1, -1, -1):
+        Z = R[:, k][:, None] + 1.0 / (1j * w * C[:, k][:, None] + 1.0 / Z)
+    return Z

@@ -75,3 +75,46 @@ def staircase(n: int) -> np.ndarray:
 def diagonals_2d_features(n: int, features: int) -> np.ndarray:
     """create_2d_diagonals(N, features) (taufactor/utils.py:196-203)."""
     return diagonals_2d(n, features)
+
+
+def graded_2d(nx: int = 250, cube_size: int = 25) -> dict:
+    """The six 2-D graded validation structures of
+    docs/notebooks/01-tortuosity.ipynb (cell 4): diagonals, diag-pillars,
+    large-small, small-large, smooth-rough, wedge; each [nx, nx, 1]."""
+    x, y, z = np.ogrid[:nx, :nx, :1]
+    fields = {}
+    fields["diagonals"] = ((x + y + z) // cube_size) % 2
+    pillars = x - x + ((y // cube_size) + (z // cube_size)) % 2
+    p = np.zeros((nx, nx, 1))
+    p[:125, :] = pillars[:125, :]
+    p[125:, :] = fields["diagonals"][:125, :]
+    fields["diag-pillars"] = np.roll(p[::-1, :, :], shift=10, axis=1)
+    p = np.zeros((nx, nx, 1))
+    p[:125, 0:40] = 1
+    p[125:, 15:25] = 1
+    p[:, 50:100] = p[:, :50]
+    p[:, 100:200] = p[:, :100]
+    p[:, 200:250] = p[:, :50]
+    p = np.roll(p, shift=5, axis=1)
+    fields["large-small"] = p
+    fields["small-large"] = p[::-1, :, :].copy()
+    p = pillars.copy()
+    p[0::2, :, :] = np.roll(p[0::2, :, :], shift=1, axis=1)
+    p[:125, :] = pillars[:125, :]
+    fields["smooth-rough"] = np.roll(p, shift=10, axis=1)
+    p = np.zeros((nx, nx, 1))
+    mask = ((x + 0.5) / nx + 4 * (y + 0.5) / nx < 2) & ((x + 0.5) / nx - 4 * (y + 0.5) / nx < 0)
+    p[mask] = 1
+    p[:, 125:] = p[:, :125][:, ::-1]
+    p[-1, :] = 0
+    fields["wedge"] = p
+    return fields
+
+
+GRADED_ORDER = ["diagonals", "diag-pillars", "large-small", "small-large", "smooth-rough", "wedge"]
+
+
+def graded_batch() -> np.ndarray:
+    """np.stack of the six structures in the notebook's order (cell 7)."""
+    f = graded_2d()
+    return np.stack([f[k] for k in GRADED_ORDER], axis=0)

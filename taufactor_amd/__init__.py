@@ -6,10 +6,11 @@ include/tauhip.h) loaded through ctypes; PyTorch-ROCm holds the device
 buffers.  There is no CPU code path.
 """
 from .batch import BatchSolver
+from .electrode import ElectrodeSolver, PeriodicElectrodeSolver
 from .autograd import effective_diffusivity
 from .solver import (AnisotropicSolver, FieldSolver, MultiPhaseSolver, PeriodicMultiPhaseSolver,
                      PeriodicSolver, SORSolver, Solver, ThroughTransportSolver)
 
 __all__ = ["Solver", "MultiPhaseSolver", "PeriodicSolver", "PeriodicMultiPhaseSolver", "AnisotropicSolver",
-           "BatchSolver", "FieldSolver", "effective_diffusivity", "SORSolver", "ThroughTransportSolver"]
+           "ElectrodeSolver", "PeriodicElectrodeSolver", "BatchSolver", "FieldSolver", "effective_diffusivity", "SORSolver", "ThroughTransportSolver"]
 __version__ = "0.1.0"

@@ -18,6 +18,8 @@ KIND_MULTIPHASE = 1
 KIND_FIELD = 2
 KIND_ANISOTROPIC = 3
 ANISO_TABLE = 147
+KIND_ELECTRODE = 4
+ELECTRODE_TABLE = 64
 SETUP_PERIODIC_YZ = 1
 SETUP_ANISOTROPIC = 2
 MAX_FUSED_HALF_SWEEPS = 6
@@ -41,6 +43,8 @@ SIGNATURES = {
                                  _vp]),
     "tf_setup_multiphase": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _vp, _vp, _vp,
                                      _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_setup_electrode": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int,
+                                    _c_int, _vp]),
     "tf_wrap_ghosts": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep_codes_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int]),
     "tf_pack_sweep_codes": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),

@@ -161,10 +161,37 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
     if (in) {
         int x = xp - 1, y = yp - 1, z = zp - 1;
         int m = m_at(x, y, z);
-        int cond = MODE == 0 ? (m != 0) : (int)cls_cond[m];
+        int cond = MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : (int)cls_cond[m];
         // mask * linspace in float32 (taufactor/taufactor.py:291)
         f = (cond ? 1.0f : 0.0f) * xinit[x];
-        if (MODE == 0 && aniso) {
+        if (MODE == 2) {
+            if (cond) {
+                // conductive neighbours with x ghosts 2 (x = 0) and 0 (x = X+1),
+                // reactive neighbours with every ghost 0; y/z wrap round when
+                // periodic (taufactor/electrode.py:47-62, 133-151)
+                auto at = [&](int xx, int yy, int zz) -> int {
+                    if (periodic) {
+                        yy = (yy + v.Y) % v.Y;
+                        zz = (zz + v.Z) % v.Z;
+                    } else if (yy < 0 || yy >= v.Y || zz < 0 || zz >= v.Z) {
+                        return 0;
+                    }
+                    return m_at(xx, yy, zz);
+                };
+                int cn = 0, rn = 0;
+                const int nb[6][3] = {{x + 1, y, z}, {x - 1, y, z}, {x, y + 1, z},
+                                      {x, y - 1, z}, {x, y, z + 1}, {x, y, z - 1}};
+                for (int k = 0; k < 6; ++k) {
+                    const int xx = nb[k][0];
+                    if (xx < 0) { cn += 2; continue; }
+                    if (xx >= v.X) continue;
+                    const int mm = at(xx, nb[k][1], nb[k][2]);
+                    cn += mm == 1;
+                    rn += mm == 2;
+                }
+                code = (uint8_t)(1 + 7 * cn + rn);
+            }
+        } else if (MODE == 0 && aniso) {
             if (cond) {
                 // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
                 // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
@@ -204,8 +231,13 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
         // ghost: x planes hold 2*bc on their y/z interior, everything else 0
         // (SORSolver._pad, taufactor/taufactor.py:234-237)
         f = 0.0f;
-        if (yz_in && xp == 0) f = -1.0f;
-        if (yz_in && xp == v.X + 1) f = 1.0f;
+        if (MODE == 2) {
+            // ElectrodeSolver pads with [2*left_bc, 0] (taufactor/electrode.py:45)
+            if (yz_in && xp == 0) f = 2.0f;
+        } else {
+            if (yz_in && xp == 0) f = -1.0f;
+            if (yz_in && xp == v.X + 1) f = 1.0f;
+        }
         if (MODE == 1 && yz_in && (xp == 0 || xp == v.X + 1)) {
             // diff_map[:,0] = diff_map[:,1] (taufactor/taufactor.py:591-592)
             int x = xp == 0 ? 0 : v.X - 1;
@@ -236,7 +268,7 @@ __global__ void k_layer_count(const uint8_t* __restrict__ mask, const uint8_t* _
     uint32_t acc = 0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int m = M[i];
-        acc += MODE == 0 ? (m != 0) : cls_cond[m];
+        acc += MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : cls_cond[m];
     }
     __shared__ uint32_t red[256];
     red[threadIdx.x] = acc;
@@ -271,6 +303,42 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
     TF_LAUNCH_CHECK("k_setup");
     hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, B), dim3(256), 0, st, mask, cls_cond, cnt, v);
     TF_LAUNCH_CHECK("k_layer_count");
+    return 0;
+}
+
+// Sum over each layer of the reactive-neighbour part of the electrode codes;
+// one block per (x, b)
+__global__ void k_reactive_count(const uint8_t* __restrict__ codes, uint32_t* __restrict__ cnt, Vol v) {
+    const int x = blockIdx.x, b = blockIdx.y;
+    const uint8_t* C = codes + (size_t)b * v.sB + (size_t)(x + 1) * v.sX;
+    uint32_t acc = 0;
+    for (int i = threadIdx.x; i < v.Y * v.Z; i += blockDim.x) {
+        const int y = i / v.Z, z = i - y * v.Z;
+        const int c = C[(y + 1) * v.sY + z + 1];
+        acc += c ? (uint32_t)((c - 1) % 7) : 0u;
+    }
+    __shared__ uint32_t red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = red[0];
+}
+
+extern "C" int tf_setup_electrode(const uint8_t* cls, const float* xinit, uint8_t* codes,
+                                  float* field_a, float* field_b, uint32_t* layer_count,
+                                  uint32_t* reactive_count, int B, int X, int Y, int Z, int flags,
+                                  void* stream) {
+    if (flags & ~TF_SETUP_PERIODIC_YZ) return fail(-1, "unknown set-up flags 0x%x", flags);
+    if (!reactive_count) return fail(-1, "null pointer argument");
+    if (int rc = setup_common<2>(cls, nullptr, xinit, codes, field_a, field_b, layer_count, B, X, Y, Z,
+                                 flags, stream))
+        return rc;
+    hipLaunchKernelGGL(k_reactive_count, dim3(X, B), dim3(256), 0, (hipStream_t)stream, codes,
+                       reactive_count, make_vol(X, Y, Z));
+    TF_LAUNCH_CHECK("k_reactive_count");
     return 0;
 }
 
@@ -606,6 +674,8 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
     } else if (KIND == TF_KIND_ANISOTROPIC) {
         if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+    } else if (KIND == TF_KIND_ELECTRODE) {
+        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
     }
     // spacing weights of TF_KIND_ANISOTROPIC (wave-uniform)
     const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
@@ -812,6 +882,18 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
                 s = qt - n.c;
                 s = s * w;
                 out = n.c + s;
+            } else if constexpr (KIND == TF_KIND_ELECTRODE) {
+                // SORSolver.sum_weighted_neighbours / factor with the reaction
+                // term in the factor (taufactor/taufactor.py:47-56, 95-103, 175-181)
+                float s = n.xp + n.xm;
+                s = s + n.yp;
+                s = s + n.ym;
+                s = s + n.zp;
+                s = s + n.zm;
+                s = s / sT[Co[q][o]];
+                s = s - n.c;
+                s = s * w;
+                out = n.c + s;
             } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
                 // ((x+ + x-) + Ky*(y+ + y-)) + Kz*(z+ + z-), / nn
                 // (AnisotropicSolver.sum_weighted_neighbours, taufactor/taufactor.py:473-478)
@@ -892,8 +974,9 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const size_t off = (size_t)b * a.v.sB;
     const size_t coff = KIND == TF_KIND_FIELD ? off
                                               : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
+    const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
     sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                 static_cast<const code_t<KIND>*>(a.codes) + coff, a.hm, a.ncls, a.v,
+                                 static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls, a.v,
                                  a.w, a.colour0, a.TX, tz, ty, tx);
 }
 
@@ -904,6 +987,7 @@ static size_t sweep_lds_bytes(int kind, int ncls) {
     size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
     bytes += kind == TF_KIND_MULTIPHASE    ? (size_t)ncls * ncls * sizeof(float)
              : kind == TF_KIND_ANISOTROPIC ? TF_ANISO_TABLE * sizeof(float)
+             : kind == TF_KIND_ELECTRODE   ? TF_ELECTRODE_TABLE * sizeof(float)
                                            : 18 * sizeof(float);
     return bytes;
 }
@@ -998,8 +1082,10 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     if (!src || !dst || !codes) return fail(-1, "null pointer argument");
     if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
     if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD &&
-        kind != TF_KIND_ANISOTROPIC)
+        kind != TF_KIND_ANISOTROPIC && kind != TF_KIND_ELECTRODE)
         return fail(-1, "unknown kind %d", kind);
+    if (kind == TF_KIND_ELECTRODE && (!hm_lut || n_classes != TF_ELECTRODE_TABLE))
+        return fail(-1, "electrode sweep needs the %d-entry prefactor table per volume", TF_ELECTRODE_TABLE);
     if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 64))
         return fail(-1, "multiphase sweep needs hm_lut and 1..64 classes (got %d)", n_classes);
     if (kind == TF_KIND_ANISOTROPIC && (!hm_lut || n_classes != TF_ANISO_TABLE))
@@ -1032,6 +1118,8 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
         return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_ANISOTROPIC)
         return dispatch_sweep<TF_KIND_ANISOTROPIC>(n_half, a, B, t.waves, t.rows_per_wave, st);
+    if (kind == TF_KIND_ELECTRODE)
+        return dispatch_sweep<TF_KIND_ELECTRODE>(n_half, a, B, t.waves, t.rows_per_wave, st);
     return dispatch_sweep<TF_KIND_FIELD>(n_half, a, B, t.waves, t.rows_per_wave, st);
 }
 
@@ -1055,7 +1143,7 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
                                                 const float* __restrict__ hm, int ncls,
                                                 const Vol v, int x, double& sum_conc,
                                                 double& sum_flux) {
-    const bool has_face = x < v.X - 1;
+    const bool has_face = KIND == TF_KIND_ELECTRODE || x < v.X - 1;
     double accC = 0.0, accF = 0.0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int y = wave; y < v.Y; y += kSumThreads / 64) {
@@ -1064,7 +1152,14 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
             const int o = row + z;
             const float c = V[o];
             accC += (double)c;
-            if (has_face) {
+            if constexpr (KIND == TF_KIND_ELECTRODE) {
+                // fluxes = -c[x] + c[x-1]; at x = 0 (left_bc - c)/0.5; zero
+                // where c or c[x-1] is 0 (taufactor/electrode.py:80-83)
+                const float cp = V[o - v.sX];
+                float d = x == 0 ? (1.0f - c) / 0.5f : -c + cp;
+                if (c == 0.0f || cp == 0.0f) d = 0.0f;
+                accF += (double)d;
+            } else if (has_face) {
                 const float cn = V[o + v.sX];
                 float d = cn - c;
                 if constexpr (KIND == TF_KIND_SOLVER) {
@@ -1112,7 +1207,10 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
     layer_sum_block<KIND>(F + (size_t)b * v.sB, C + (size_t)b * v.sB, hm, ncls, v, x, sc, sf);
     if (threadIdx.x == 0) {
         conc[(size_t)b * v.X + x] = sc;
-        if (x < v.X - 1) flux[(size_t)b * (v.X - 1) + x] = sf;
+        if (KIND == TF_KIND_ELECTRODE)
+            flux[(size_t)b * v.X + x] = sf;
+        else if (x < v.X - 1)
+            flux[(size_t)b * (v.X - 1) + x] = sf;
     }
 }
 
@@ -1136,6 +1234,9 @@ extern "C" int tf_layer_sums(const float* field, const void* codes, const float*
     else if (kind == TF_KIND_FIELD)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_FIELD>, grid, dim3(kSumThreads), 0, st, field,
                            static_cast<const float*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
+    else if (kind == TF_KIND_ELECTRODE)
+        hipLaunchKernelGGL(k_layer_sums<TF_KIND_ELECTRODE>, grid, dim3(kSumThreads), 0, st, field,
+                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
     else if (kind == TF_KIND_ANISOTROPIC && hm_lut && n_classes == TF_ANISO_TABLE)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_ANISOTROPIC>, grid, dim3(kSumThreads), 0, st, field,
                            static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);

@@ -236,7 +236,9 @@ class SORSolver(ABC):
             self._n_classes, self.kind, B, X, Y, Z,
             _lib.ptr(self._flux_sum), _lib.ptr(self._conc_sum), self._stream())
         _lib.check(st, "tf_layer_sums")
-        flux = self._flux_sum.cpu().numpy()[:, : X - 1]
+        # X - 1 internal faces; the electrode fluxes also have the left face
+        width = X if self.kind == _lib.KIND_ELECTRODE else X - 1
+        flux = self._flux_sum.cpu().numpy()[:, :width]
         conc = self._conc_sum.cpu().numpy()
         return flux, conc
 

@@ -1,9 +1,11 @@
-"""Synthetic benchmark structures (taufactor/utils.py:101-147, 179-221).
+"""Synthetic benchmark structures and the transmission-line impedance
+(taufactor/utils.py:75-147, 179-221).
 
 The generators the reference's benchmark harness registers (fcc packing,
 stacked blocks, 2-D and 3-D diagonal channels, zigzag channels), with the
 reference's names, arguments and integer output, so that harness rows are
-computed on the same voxels.
+computed on the same voxels; and the impedance of a resistor/capacitor
+ladder that ElectrodeSolver turns into its tortuosity.
 """
 from __future__ import annotations
 
@@ -71,3 +73,31 @@ def create_3d_diagonals(Nx, features=1):
     size = _feature_size(Nx, features)
     x, y, z = np.ogrid[:Nx, :Nx, :Nx]
     return (((x + y + z) // size) % 2).astype(int)
+
+
+def compute_impedance(R, C, freq):
+    """Input impedance of a transmission line of resistors R and capacitors C
+    (excitation 1 at the left end, closed right end), per frequency
+    (taufactor/utils.py:75-84): Z <- R_i + 1/(j w C_i + 1/Z) from the right."""
+    Z = np.full_like(freq, 1e50, dtype=complex)
+    for R_i, C_i in zip(R[::-1], C[::-1]):
+        Z = R_i + 1.0 / (1j * freq * C_i + 1.0 / Z)
+    return Z
+
+
+def compute_impedance_batched(R, C, freq):
+    """compute_impedance for R, C of shape (bs, Nx) and freq (F,) or (bs, F);
+    returns (bs, F) (taufactor/utils.py:87-98)."""
+    if R.shape != C.shape:
+        raise ValueError(f"R and C must have same shape; got {R.shape} vs {C.shape}")
+    bs, Nx = R.shape
+    if freq.ndim == 1:
+        w = freq[None, :].repeat(bs, axis=0)
+    elif freq.ndim == 2 and freq.shape[0] == bs:
+        w = freq
+    else:
+        raise ValueError("freq must be (F,) or (bs, F)")
+    Z = np.full_like(w, 1e50, dtype=complex)
+    for k in range(Nx - 1, -1, -1):
+        Z = R[:, k][:, None] + 1.0 / (1j * w * C[:, k][:, None] + 1.0 / Z)
+    return Z
