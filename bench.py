@@ -133,7 +133,9 @@ def main(argv=None) -> int:
         s.tile = tuple(int(v) for v in args.tile.split(","))
     t0 = time.perf_counter()
     H, tile = s._launch_config()  # runs the autotuner unless both are given
+    _, rtile, n_tiles = s.resolved_launch()
     log(f"launch configuration: {H} half-sweeps per launch, tile {tile or 'library default'} "
+        f"-> {rtile[0]}x{rtile[1]} waves x rows, x tile {rtile[2]}, {n_tiles} tiles "
         f"({'autotuned' if args.half_sweeps is None or args.tile is None else 'given'}, "
         f"{time.perf_counter() - t0:.2f} s)")
     if 100 % H:
@@ -188,7 +190,8 @@ def main(argv=None) -> int:
             "tau": float(s.tau[0]),
             "D_eff": float(s.D_eff[0]),
             "half_sweeps_per_pass": H,
-            "tile": list(tile) if tile else "default",
+            "tile": list(rtile),
+            "tiles_per_launch": n_tiles,
             "launch_config": "autotuned" if args.half_sweeps is None or args.tile is None else "given",
             "volume_sha256": workloads.sha256(img),
         },
@@ -209,7 +212,7 @@ def main(argv=None) -> int:
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if t.get("half_sweeps_per_pass") == H and t.get("tile") == (list(tile) if tile else "default"):
+        if t.get("half_sweeps_per_pass") == H and t.get("tile") == list(rtile):
             out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
             out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch, "
                                                  "profiles/k_sweep_solver_%d_traffic.md"

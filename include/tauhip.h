@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 3
+#define TF_ABI_VERSION 4
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -65,8 +65,10 @@ extern "C" {
  * (TZ = 120 for n_half <= 4, 112 for 5 and 6) along tx planes of x; one
  * wavefront per row.  Compiled shapes (waves x rows_per_wave): 8x2, 4x4,
  * 12x2, 8x4, 16x2.  Zero fields select the defaults: 16x2 for one-byte
- * codes with n_half <= 4 and Y >= 64, else 8x2 (8x2 for batches); tx 64,
- * shortened (>= 8) for small volumes to keep about 512 workgroups. */
+ * codes with n_half <= 4 and Y >= 64, else 8x2 (8x2 for batches).  tx 0
+ * (tf_sweep) lets the launch choose the x-tile length: the one whose tiles
+ * fill the device's resident workgroup slots in the fewest, shortest
+ * rounds; batches default to tx 64. */
 typedef struct tf_tile {
     int waves, rows_per_wave, tx;
 } tf_tile;
@@ -191,6 +193,13 @@ int tf_sweep(const float* src, float* dst, const void* codes,
              int B, int X, int Y, int Z, float omega, int first_colour,
              int n_half, const tf_tile* tile, void* stream);
 
+/* The launch configuration tf_sweep would use for these arguments: the
+ * workgroup shape and the x-tile length after defaults (tx 0 resolved on
+ * the current device), and the number of tiles.  Launches nothing; no
+ * counterpart in the reference (the launch shape is this library's). */
+int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int Y, int Z,
+                           int n_half, const tf_tile* tile, tf_tile* out, int* n_tiles);
+
 /* One half-sweep of colour `colour` in place, one thread per voxel.  The
  * unblocked form of tf_sweep, kept as the simplest statement of the update
  * for tests; same arithmetic.                                            */
@@ -275,7 +284,8 @@ typedef struct tf_volume_state {
     double tau, old_tau, D_rel, rel_err;
 } tf_volume_state;
 
-/* Number of workgroup tiles tf_sweep / tf_batch_sweep use for one volume. */
+/* Number of workgroup tiles tf_batch_sweep uses for one volume (tile
+ * defaults of batches: 8x2, tx 64). */
 int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_tile* tile, int* count);
 
 /* n_half half-sweeps on every active volume of a batch, one launch.  All

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -58,6 +58,9 @@ SIGNATURES = {
     "tf_setup_field": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_grad_field": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "tf_grad_classes": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
+    "tf_sweep_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                        ctypes.POINTER(Tile), ctypes.POINTER(Tile),
+                                        ctypes.POINTER(_c_int)]),
     "tf_sweep_tile_count": (_c_int, [_c_int, _c_int, _c_int, _c_int, ctypes.POINTER(Tile),
                                      ctypes.POINTER(ctypes.c_int)]),
     "tf_batch_sweep": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _c_int, _c_int,

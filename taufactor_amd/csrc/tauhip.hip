@@ -590,6 +590,7 @@ struct SweepArgs {
     int colour0;
     int TX;
     int nTz, nTy, nTx, total;
+    int resolve_only;  // set TX and the tile counts, launch nothing
 };
 
 // compile-time loop: f(std::integral_constant<int, I>{}) for I in [B, E)
@@ -992,6 +993,55 @@ static size_t sweep_lds_bytes(int kind, int ncls) {
     return bytes;
 }
 
+// Workgroups of one sweep kernel resident on the current device at once
+// (CUs x workgroups per CU); 0 if the runtime cannot say.  `cache` holds the
+// answer + 1 per device (0: not asked yet).
+constexpr int kMaxDevices = 64;
+static int resident_workgroups(const void* fn, int threads, size_t lds, int* cache) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    if (cache[dev] > 0) return cache[dev] - 1;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = per_cu = 0;
+    }
+    cache[dev] = std::max(0, cus) * std::max(0, per_cu) + 1;
+    return cache[dev] - 1;
+}
+
+// x-tile length that minimises the modelled duration of one launch: the
+// tiles are spread evenly over the 8 XCDs (k_sweep's numbering) and run in
+// rounds of the resident workgroups; a tile costs its streamed planes,
+// TX + 2(H-1) rounded up to the unrolled step count NP, plus a fixed start.
+// A grid with fewer tiles than resident slots is one round, so for small
+// volumes the shortest tile that still fits one round wins.
+static int balanced_tx(int X, long long yz_tiles, int H, int NP, int slots) {
+    if (slots <= 0) return std::min(X, 64);
+    const long long per_xcd = std::max(1, slots / 8);
+    long long best_cost = -1;
+    int best = X;
+    for (int n = 1; n <= X; ++n) {
+        const int tx = (X + n - 1) / n;
+        const int nx = (X + tx - 1) / tx;
+        if (nx != n) continue;  // same tiling as a smaller n
+        const long long tiles = yz_tiles * nx;
+        const long long rounds = ((tiles + 7) / 8 + per_xcd - 1) / per_xcd;
+        const long long steps = (tx + 2 * (H - 1) + NP - 1) / NP * NP + 4;
+        const long long cost = rounds * steps;
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = tx;
+        }
+        if (tx <= 8) break;
+    }
+    return best;
+}
+
 template <int H, int KIND, int NW, int RPW>
 static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
     using Sh = SweepShape<H, NW, RPW>;
@@ -1001,13 +1051,6 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
         if (lds > 160 * 1024)
             return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
-        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
-        a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
-        a.nTx = (a.v.X + a.TX - 1) / a.TX;
-        long long total = (long long)a.nTz * a.nTy * a.nTx * B;
-        if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
-        a.total = (int)total;
-        const int grid = (int)(((total + 7) / 8) * 8);
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
             // opt in to more than 64 KiB of dynamic LDS; if the runtime
@@ -1017,6 +1060,20 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
                 (void)hipGetLastError();
             attr_set = true;
         }
+        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
+        a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
+        if (a.TX <= 0) {
+            static int slots_cache[kMaxDevices] = {};
+            const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
+                                                  lds, slots_cache);
+            a.TX = balanced_tx(a.v.X, (long long)a.nTz * a.nTy * B, H, Sh::NP, slots);
+        }
+        a.nTx = (a.v.X + a.TX - 1) / a.TX;
+        long long total = (long long)a.nTz * a.nTy * a.nTx * B;
+        if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
+        a.total = (int)total;
+        if (a.resolve_only) return 0;
+        const int grid = (int)(((total + 7) / 8) * 8);
         hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
         TF_LAUNCH_CHECK("k_sweep");
         return 0;
@@ -1054,33 +1111,21 @@ static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream
 // owned voxels of z per tile (SweepShape::TZ)
 static int tile_tz(int H) { return ((kRowLen - 4 * ((H + 3) / 4) - H) / 4) * 4; }
 
-// Default workgroup shape and x-tile length.  32-row tiles (16 waves x 2)
-// halve the y-halo overhead of 16-row tiles where their LDS ring fits
-// (one-byte codes, n_half <= 4) and the volume is tall enough; the x tile
-// is shortened for small volumes so that the grid still has about two
-// workgroups per CU.  Any shape gives identical results.
-static void default_tile(int H, int kind, int B, int X, int Y, int Z, tf_tile* t) {
+// Default workgroup shape.  32-row tiles (16 waves x 2) halve the y-halo
+// overhead of 16-row tiles where their LDS ring fits (one-byte codes,
+// n_half <= 4) and the volume is tall enough; the x-tile length is left to
+// the launch (balanced_tx).  Any shape gives identical results.
+static void default_tile(int H, int kind, int Y, tf_tile* t) {
     const bool tall = kind != TF_KIND_FIELD && H <= 4 && Y >= 64;
     t->waves = tall ? 16 : 8;
     t->rows_per_wave = 2;
-    const long long ty = (long long)t->waves * t->rows_per_wave - 2 * H;
-    const long long tz = tile_tz(H);
-    const long long yz = (long long)B * ((Y + ty - 1) / ty) * ((Z + tz - 1) / tz);
-    const long long want = 512;  // workgroups
-    long long tx = 64;
-    if (yz * ((X + 63) / 64) < want) {
-        const long long nx = (want + yz - 1) / yz;
-        tx = std::max(8LL, (X + nx - 1) / nx);
-    }
-    t->tx = (int)std::min(tx, 64LL);
+    t->tx = 0;  // chosen at launch from the resident workgroups (balanced_tx)
 }
 
-extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const float* hm_lut,
-                        int n_classes, int kind, int B, int X, int Y, int Z, float omega,
-                        int first_colour, int n_half, const tf_tile* tile, void* stream) {
+// Checks shared by tf_sweep and tf_sweep_launch_config; fills the shape.
+static int sweep_prepare(const float* hm_lut, int n_classes, int kind, int B, int X, int Y, int Z,
+                         int n_half, const tf_tile* tile, tf_tile* t) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
-    if (!src || !dst || !codes) return fail(-1, "null pointer argument");
-    if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
     if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD &&
         kind != TF_KIND_ANISOTROPIC && kind != TF_KIND_ELECTRODE)
         return fail(-1, "unknown kind %d", kind);
@@ -1090,28 +1135,24 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
         return fail(-1, "multiphase sweep needs hm_lut and 1..64 classes (got %d)", n_classes);
     if (kind == TF_KIND_ANISOTROPIC && (!hm_lut || n_classes != TF_ANISO_TABLE))
         return fail(-1, "anisotropic sweep needs the %d-entry prefactor table", TF_ANISO_TABLE);
-    if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
         return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
-    tf_tile t;
-    default_tile(n_half, kind, B, X, Y, Z, &t);
+    default_tile(n_half, kind, Y, t);
     if (tile) {
-        if (tile->waves > 0) t.waves = tile->waves;
-        if (tile->rows_per_wave > 0) t.rows_per_wave = tile->rows_per_wave;
-        if (tile->tx > 0) t.tx = tile->tx;
+        if (tile->waves > 0) t->waves = tile->waves;
+        if (tile->rows_per_wave > 0) t->rows_per_wave = tile->rows_per_wave;
+        if (tile->tx > 0) t->tx = tile->tx;
     }
-    if (t.tx < 1 || t.tx > 65536) return fail(-1, "invalid tile tx=%d", t.tx);
+    if (t->tx < 0 || t->tx > 65536) return fail(-1, "invalid tile tx=%d", t->tx);
     if ((long long)(X + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64 ||
         sweep_code_volume(X, Y, Z) >= (1LL << 31))
         return fail(-1, "padded volume of %lld bytes: the sweep addresses at most 2 GiB per volume",
                     (long long)(X + 2) * (Y + 2) * (Z + 2) * 4);
-    SweepArgs a;
-    a.src = src; a.dst = dst; a.codes = codes; a.hm = hm_lut; a.ncls = n_classes;
-    a.v = make_vol(X, Y, Z);
-    a.w = omega;
-    a.colour0 = first_colour;
-    a.TX = t.tx;
-    hipStream_t st = (hipStream_t)stream;
+    return 0;
+}
+
+static int sweep_dispatch(int kind, int n_half, SweepArgs& a, int B, const tf_tile& t,
+                          hipStream_t st) {
     if (kind == TF_KIND_SOLVER)
         return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_MULTIPHASE)
@@ -1121,6 +1162,44 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     if (kind == TF_KIND_ELECTRODE)
         return dispatch_sweep<TF_KIND_ELECTRODE>(n_half, a, B, t.waves, t.rows_per_wave, st);
     return dispatch_sweep<TF_KIND_FIELD>(n_half, a, B, t.waves, t.rows_per_wave, st);
+}
+
+extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const float* hm_lut,
+                        int n_classes, int kind, int B, int X, int Y, int Z, float omega,
+                        int first_colour, int n_half, const tf_tile* tile, void* stream) {
+    tf_tile t;
+    if (int rc = sweep_prepare(hm_lut, n_classes, kind, B, X, Y, Z, n_half, tile, &t)) return rc;
+    if (!src || !dst || !codes) return fail(-1, "null pointer argument");
+    if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
+    if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
+    SweepArgs a = {};
+    a.src = src; a.dst = dst; a.codes = codes; a.hm = hm_lut; a.ncls = n_classes;
+    a.v = make_vol(X, Y, Z);
+    a.w = omega;
+    a.colour0 = first_colour;
+    a.TX = t.tx;
+    return sweep_dispatch(kind, n_half, a, B, t, (hipStream_t)stream);
+}
+
+extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int Y, int Z,
+                                      int n_half, const tf_tile* tile, tf_tile* out,
+                                      int* n_tiles) {
+    if (!out) return fail(-1, "null pointer argument");
+    // the tables are not read here: any non-null pointer passes the checks
+    static const float dummy = 0.0f;
+    tf_tile t;
+    if (int rc = sweep_prepare(&dummy, n_classes, kind, B, X, Y, Z, n_half, tile, &t)) return rc;
+    SweepArgs a = {};
+    a.v = make_vol(X, Y, Z);
+    a.ncls = n_classes;
+    a.TX = t.tx;
+    a.resolve_only = 1;
+    if (int rc = sweep_dispatch(kind, n_half, a, B, t, nullptr)) return rc;
+    out->waves = t.waves;
+    out->rows_per_wave = t.rows_per_wave;
+    out->tx = a.TX;
+    if (n_tiles) *n_tiles = a.total;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -21,6 +21,8 @@ Differences from the reference, all deliberate:
 """
 from __future__ import annotations
 
+import ctypes
+
 import math
 import warnings
 from abc import ABC, abstractmethod
@@ -189,6 +191,19 @@ class SORSolver(ABC):
         H = int(H) if H is not None else (aH or DEFAULT_HALF_SWEEPS_PER_PASS)
         tile = tuple(tile) if tile is not None else atile
         return H, tile
+
+    def resolved_launch(self):
+        """(H, (waves, rows_per_wave, tx), n_tiles) of the next sweep launches
+        with every default resolved by the library (tf_sweep_launch_config)."""
+        H, tile = self._launch_config()
+        H = max(1, min(H, _lib.MAX_FUSED_HALF_SWEEPS))
+        out, n = _lib.Tile(), ctypes.c_int(0)
+        req = _lib.Tile(*tile) if tile is not None else None
+        st = self._lib.tf_sweep_launch_config(self.kind, self._n_classes, self.batch_size, self.Nx,
+                                              self.Ny, self.Nz, H, req, ctypes.byref(out),
+                                              ctypes.byref(n))
+        _lib.check(st, "tf_sweep_launch_config")
+        return H, (out.waves, out.rows_per_wave, out.tx), n.value
 
     def _advance(self, n_half: int) -> None:
         """Run n_half reference iterations (half-sweeps) on the device."""

@@ -97,3 +97,27 @@ def test_c5_crop24_voxel_gradient():
     got = np.array([grad[tuple(v)] for v in g["voxels"]])
     fd = np.array(g["grad_fd"])
     assert np.max(np.abs(got - fd)) <= 1e-3 * np.max(np.abs(fd)), np.max(np.abs(got - fd))
+
+
+def test_balanced_x_tile_at_512():
+    """tx 0: the launch picks the x-tile length whose tiles fill the resident
+    workgroup slots in the fewest, shortest rounds.  A 16x2 workgroup at
+    n_half = 3 needs 123 KiB of LDS, so one is resident per CU; 512^3 has
+    5 x 20 tiles in z and y, and 5 x tiles of 103 planes (500 tiles, two
+    rounds of 256) beat 4 of 128 (400 tiles, also two rounds) and 2 of 256."""
+    import ctypes
+
+    from taufactor_amd import _lib
+
+    if torch.cuda.get_device_properties(0).multi_processor_count != 256:
+        pytest.skip("expected tile counts are for a 256-CU device")
+    lib = _lib.load()
+    out, n = _lib.Tile(), ctypes.c_int(0)
+    st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 0),
+                                    ctypes.byref(out), ctypes.byref(n))
+    assert st == 0, lib.tf_last_error()
+    assert (out.waves, out.rows_per_wave, out.tx, n.value) == (16, 2, 103, 500)
+    # an explicit x tile is kept
+    st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 64),
+                                    ctypes.byref(out), ctypes.byref(n))
+    assert st == 0 and out.tx == 64 and n.value == 800
