@@ -961,6 +961,283 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     }
 }
 
+// ---------------------------------------------------------------------------
+// register-blocked sweep for the kinds whose update reads only the voxel's
+// own code (Solver, anisotropic, electrode)
+// ---------------------------------------------------------------------------
+// Same compute mapping as sweep_tile (a lane holds the z-pair zl, zl+1 of
+// each of its rows; the red/black pattern is wave-uniform), but a wave owns
+// RPW consecutive rows and keeps every plane of the ring in registers:
+// x neighbours and the y neighbours inside the wave are registers, the z
+// neighbour across lanes comes by a DPP wave shift, and only the first and
+// last row of each wave go through LDS.  Half-sweep h of step s reads the
+// edge rows that the neighbouring waves published at step s-1 (the values
+// of plane s-h after half-sweep h-1, or as loaded for h = 0), double
+// buffered by step parity, so one barrier per step suffices.
+__host__ __device__ constexpr bool own_code_kind(int kind) {
+    return kind == TF_KIND_SOLVER || kind == TF_KIND_ANISOTROPIC || kind == TF_KIND_ELECTRODE;
+}
+
+template <int H, int NW, int RPW>
+struct RegShape {
+    static constexpr int EY = NW * RPW;
+    static constexpr int TY = EY - 2 * H;
+    static constexpr int ML = H + (H & 1);                    // z0 - ew: even, so ew is odd
+    static constexpr int TZ = ((kRowLen - ML - H) / 2) * 2;   // owned voxels of z (even)
+    static constexpr int NR = ((H + 5) / 2) * 2;              // planes in registers (even)
+    static constexpr int D = NR - H - 2;                      // planes in flight (2 or 3)
+    static constexpr int TBL = 640;                           // LDS bytes for the table
+    static constexpr int EB = (NW + 2) * 2 * kRowPairs;       // floats per edge buffer
+    static constexpr size_t lds_bytes() { return TBL + (size_t)H * 2 * EB * sizeof(float); }
+};
+
+// (the component goes through a scalar first: hipcc 7.2 compiles
+// __builtin_bit_cast of an ext_vector component as a cast of component 0)
+__device__ __forceinline__ float lo_f(u32x2_t v) {
+    const uint32_t t = v.x;
+    return __builtin_bit_cast(float, t);
+}
+__device__ __forceinline__ float hi_f(u32x2_t v) {
+    const uint32_t t = v.y;
+    return __builtin_bit_cast(float, t);
+}
+// lane l receives lane l-1's value (lane 0: 0) / lane l+1's value (lane 63: 0)
+__device__ __forceinline__ float from_prev_lane(uint32_t v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_next_lane(uint32_t v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false));
+}
+
+// Tile geometry requirements (checked by the launch): TX and TY even, so that
+// with the unclipped stream start S0 = x0 - (H-1) the active element of a
+// lane depends only on the launch's colour and the step and row parities.
+template <int H, int KIND, int NW, int RPW>
+__device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
+                                            const uint8_t* __restrict__ Cg,
+                                            const float* __restrict__ hm, const Vol v, float w,
+                                            int colour0, int TX, int tz, int ty, int tx) {
+    using Sh = RegShape<H, NW, RPW>;
+    constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
+    static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
+    static_assert(RPW % 2 == 0, "rows per wave must be even");
+    extern __shared__ __align__(16) unsigned char smem[];
+    float* const sT = reinterpret_cast<float*>(smem);
+    float* const sE = reinterpret_cast<float*>(smem + Sh::TBL);
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
+    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
+    const int ey0 = y0 - H;
+    const int ew = z0 - ML;        // odd
+    const int zl = ew + 2 * lane;  // z of the first voxel of this lane's pair
+    const int lx1 = min(x1 + H, v.X + 2);
+
+    if (KIND == TF_KIND_SOLVER) {
+        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
+    } else if (KIND == TF_KIND_ANISOTROPIC) {
+        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+    } else {
+        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+    }
+    const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
+    const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
+
+    // ---- global memory (see sweep_tile for the descriptor conventions) ----
+    // loads: one 8-byte field load (the lane's pair) and one 2-byte code
+    // load per row and plane, straight into the register ring
+    constexpr int kOff = 0x7fffff00;
+    const int plane_f = v.sX * 4;
+    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
+                                (uint32_t)(v.sB * 4) + 2u * kSlackBytes);
+    const int row_c = sweep_code_row(v.Z);
+    const int plane_c = (v.Y + 2) * row_c;
+    const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
+    int vf[RPW], vc[RPW];
+    bool own_row[RPW];
+    int ys[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j) {
+        const int y = ey0 + wave * RPW + j;
+        const bool rok = y >= 0 && y <= v.Y + 1;
+        vf[j] = rok ? kSlackBytes + (y * v.sY + zl) * 4 : kOff;
+        vc[j] = rok ? y * row_c + zl + 3 : kOff;
+        own_row[j] = y >= y0 && y < y1;
+        ys[j] = own_row[j] ? y : 0;
+    }
+    u32x2_t F[NR][RPW];
+    uint32_t K[NR][RPW];
+    // slots the prologue does not load hold planes before the stream start
+    // in the first steps: defined values, read by no active half-sweep
+#pragma unroll
+    for (int i = 0; i < NR; ++i)
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            F[i][j] = u32x2_t{0u, 0u};
+            K[i][j] = 0u;
+        }
+    auto issue_load = [&](int p, auto slot_c) {
+        constexpr int sl = decltype(slot_c)::value;
+        const bool pok = p >= 0 && p < lx1;
+        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, pok ? vf[j] : kOff, sf, 0);
+            K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
+        }
+    };
+    const int zs = zl >= z0 ? zl * 4 : kOff;
+    auto store_plane = [&](int q, auto slot_c) {
+        constexpr int sl = decltype(slot_c)::value;
+        const bool qok = q >= x0 && q < x1;
+        const int qs = qok ? q : 0;
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) {
+            const rsrc_t r = make_rsrc(D + qs * v.sX + ys[j] * v.sY,
+                                       (qok && own_row[j]) ? (uint32_t)(4 * z1) : 0u);
+            __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, zs, 0, 0);
+        }
+    };
+
+    // ---- per-tile bounds of each half-sweep (planes only: rows and z are
+    // updated unconditionally; outside a half-sweep's region the result is
+    // never read by a later half-sweep of this pass nor stored, and ghosts
+    // (value 0 with code 0, whose table entry gives a zero quotient) keep
+    // their value) ----
+    int plo[H], pn[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+        const int grow = H - 1 - h;
+        plo[h] = max(x0 - grow, 1);
+        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
+    }
+    // unclipped stream start: with TX and TY even, S0 + ey0 is odd, so the
+    // active element of row j at step u of an unrolled block,
+    // (s + y + colour0) & 1, is A ^ ((u + j) & 1) with A = colour0 ^ 1 for
+    // every tile of the launch; the parity of S0 is that of H
+    const int S0 = x0 - (H - 1);
+    const int S1 = (x1 - 1) + (H - 1);
+    const int A = (colour0 & 1) ^ 1;
+
+    // edge buffers: [h][parity][NW + 2][top, bottom][64 lanes]; wave w
+    // publishes in slot w + 1, so the missing neighbours of the first and
+    // last wave read a slot nobody writes (their rows are halo rows)
+    float* const eb = sE + (wave + 1) * 2 * kRowPairs + lane;
+    // parity of step s is that of S0 + u, S0 having the parity of H
+    auto ebuf = [&](int h, int par) { return eb + (h * 2 + par) * EB; };
+
+    auto body = [&](auto a_c) {
+        constexpr int AA = decltype(a_c)::value;
+        // active element (0: z = zl, 1: z = zl + 1) of row j at step u
+        auto act = [](int u, int j) { return (AA ^ (u + j)) & 1; };
+        auto par = [](int u) { return (H + u) & 1; };
+
+        auto publish = [&](int h, int u, auto slot_c) {
+            constexpr int sl = decltype(slot_c)::value;
+            float* e = ebuf(h, par(u));
+            const int a0 = act(u, 0), a1 = act(u, RPW - 1);
+            e[0] = a0 ? hi_f(F[sl][0]) : lo_f(F[sl][0]);
+            e[kRowPairs] = a1 ? hi_f(F[sl][RPW - 1]) : lo_f(F[sl][RPW - 1]);
+        };
+
+        auto stage = [&](auto h_c, auto u_c, int p) {
+            constexpr int h = decltype(h_c)::value, u = decltype(u_c)::value;
+            constexpr int sl = ((u - h + 1) % NR + NR) % NR;
+            constexpr int slm = (sl + NR - 1) % NR, slp = (sl + 1) % NR;
+            if ((unsigned)(p - plo[h]) >= (unsigned)pn[h]) return;
+            const float* e = ebuf(h, par(u + 1));
+            const float ym0 = e[-kRowPairs];        // bottom row of the wave above
+            const float ypL = e[2 * kRowPairs];     // top row of the wave below
+#pragma unroll
+            for (int j = 0; j < RPW; ++j) {
+                const int a = act(u, j);
+                const u32x2_t m = F[sl][j];
+                const float c = a ? hi_f(m) : lo_f(m);
+                const float xm = a ? hi_f(F[slm][j]) : lo_f(F[slm][j]);
+                const float xp = a ? hi_f(F[slp][j]) : lo_f(F[slp][j]);
+                const float ym = j > 0 ? (a ? hi_f(F[sl][j - 1]) : lo_f(F[sl][j - 1])) : ym0;
+                const float yp = j < RPW - 1 ? (a ? hi_f(F[sl][j + 1]) : lo_f(F[sl][j + 1])) : ypL;
+                const float zp = a ? from_next_lane(m.x) : hi_f(m);
+                const float zm = a ? lo_f(m) : from_prev_lane(m.y);
+                const uint32_t code = a ? (K[sl][j] >> 8) : (K[sl][j] & 0xffu);
+                float out;
+                if constexpr (KIND == TF_KIND_SOLVER) {
+                    const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
+                    float s = xp + xm;
+                    s = s + yp;
+                    s = s + ym;
+                    s = s + zp;
+                    s = s + zm;
+                    float qt = s * rdv.x;
+                    const float e2 = __builtin_fmaf(-qt, rdv.y, s);
+                    qt = __builtin_fmaf(e2, rdv.x, qt);
+                    if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
+                        qt = s / (code ? (float)code : __builtin_inff());
+                    s = qt - c;
+                    s = s * w;
+                    out = c + s;
+                } else if constexpr (KIND == TF_KIND_ELECTRODE) {
+                    float s = xp + xm;
+                    s = s + yp;
+                    s = s + ym;
+                    s = s + zp;
+                    s = s + zm;
+                    s = s / sT[code];
+                    s = s - c;
+                    s = s * w;
+                    out = c + s;
+                } else {
+                    float s = xp + xm;
+                    float t = yp + ym;
+                    t = ky * t;
+                    s = s + t;
+                    t = zp + zm;
+                    t = kz * t;
+                    s = s + t;
+                    s = s / sT[code];
+                    s = s - c;
+                    s = s * w;
+                    out = c + s;
+                }
+                if (a)
+                    F[sl][j].y = __builtin_bit_cast(uint32_t, out);
+                else
+                    F[sl][j].x = __builtin_bit_cast(uint32_t, out);
+            }
+        };
+
+        // planes S0-1 .. S0+1+DP in flight; plane S0's edges published as
+        // "step S0-1" for the first half-sweep
+        static_for<0, DP + 3>([&](auto i_c) {
+            constexpr int i = decltype(i_c)::value;
+            issue_load(S0 - 1 + i, ic<i>{});
+        });
+        publish(0, -1 + NR, ic<1>{});  // u = -1 has the parities of u = NR - 1
+        __syncthreads();
+
+        for (int s0 = S0; s0 <= S1; s0 += NR) {
+            static_for<0, NR>([&](auto u_c) {
+                constexpr int u = decltype(u_c)::value;
+                const int s = s0 + u;
+                static_for<0, H>([&](auto h_c) {
+                    constexpr int h = decltype(h_c)::value;
+                    stage(h_c, u_c, s - h);
+                    if constexpr (h + 1 < H) publish(h + 1, u, ic<((u - h + 1) % NR + NR) % NR>{});
+                });
+                publish(0, u, ic<(u + 2) % NR>{});
+                store_plane(s - (H - 1), ic<((u - H + 2) % NR + NR) % NR>{});
+                issue_load(s + 2 + DP, ic<((u - H + 1) % NR + NR) % NR>{});
+                __syncthreads();
+            });
+        }
+    };
+    if (A)
+        body(ic<1>{});
+    else
+        body(ic<0>{});
+}
+
 template <int H, int KIND, int NW, int RPW>
 __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     // XCD-aware tile numbering (grid is a multiple of 8)
@@ -976,13 +1253,29 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const size_t coff = KIND == TF_KIND_FIELD ? off
                                               : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
-    sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                 static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls, a.v,
-                                 a.w, a.colour0, a.TX, tz, ty, tx);
+    if constexpr (own_code_kind(KIND))
+        rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
+                                      static_cast<const uint8_t*>(a.codes) + coff, hm, a.v, a.w,
+                                      a.colour0, a.TX, tz, ty, tx);
+    else
+        sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
+                                     static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls,
+                                     a.v, a.w, a.colour0, a.TX, tz, ty, tx);
 }
+
+// Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows
+// and z voxels per tile, steps per unrolled block, LDS bytes.
+template <int H, int KIND, int NW, int RPW>
+struct SweepGeom {
+    static constexpr bool reg = own_code_kind(KIND);
+    static constexpr int TY = reg ? RegShape<H, NW, RPW>::TY : SweepShape<H, NW, RPW>::TY;
+    static constexpr int TZ = reg ? RegShape<H, NW, RPW>::TZ : SweepShape<H, NW, RPW>::TZ;
+    static constexpr int NP = reg ? RegShape<H, NW, RPW>::NR : SweepShape<H, NW, RPW>::NP;
+};
 
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
+    if (own_code_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes();
     using Sh = SweepShape<H, NW, RPW>;
     const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
     size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
@@ -1020,13 +1313,14 @@ static int resident_workgroups(const void* fn, int threads, size_t lds, int* cac
 // TX + 2(H-1) rounded up to the unrolled step count NP, plus a fixed start.
 // A grid with fewer tiles than resident slots is one round, so for small
 // volumes the shortest tile that still fits one round wins.
-static int balanced_tx(int X, long long yz_tiles, int H, int NP, int slots) {
-    if (slots <= 0) return std::min(X, 64);
+static int balanced_tx(int X, long long yz_tiles, int H, int NP, int slots, bool even) {
+    auto fit = [even](int tx) { return even ? (tx + 1) & ~1 : tx; };
+    if (slots <= 0) return fit(std::min(X, 64));
     const long long per_xcd = std::max(1, slots / 8);
     long long best_cost = -1;
-    int best = X;
+    int best = fit(X);
     for (int n = 1; n <= X; ++n) {
-        const int tx = (X + n - 1) / n;
+        const int tx = fit((X + n - 1) / n);
         const int nx = (X + tx - 1) / tx;
         if (nx != n) continue;  // same tiling as a smaller n
         const long long tiles = yz_tiles * nx;
@@ -1044,8 +1338,8 @@ static int balanced_tx(int X, long long yz_tiles, int H, int NP, int slots) {
 
 template <int H, int KIND, int NW, int RPW>
 static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
-    using Sh = SweepShape<H, NW, RPW>;
-    if constexpr (Sh::TY < 1) {
+    using Sh = SweepGeom<H, KIND, NW, RPW>;
+    if constexpr (Sh::TY < (Sh::reg ? 2 : 1)) {
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
     } else {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
@@ -1066,8 +1360,10 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
             static int slots_cache[kMaxDevices] = {};
             const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
                                                   lds, slots_cache);
-            a.TX = balanced_tx(a.v.X, (long long)a.nTz * a.nTy * B, H, Sh::NP, slots);
+            a.TX = balanced_tx(a.v.X, (long long)a.nTz * a.nTy * B, H, Sh::NP, slots, Sh::reg);
         }
+        // the register-blocked kernel needs an even x tile (rsweep_tile)
+        if (Sh::reg) a.TX = (a.TX + 1) & ~1;
         a.nTx = (a.v.X + a.TX - 1) / a.TX;
         long long total = (long long)a.nTz * a.nTy * a.nTx * B;
         if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
@@ -1108,8 +1404,8 @@ static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream
     }
 }
 
-// owned voxels of z per tile (SweepShape::TZ)
-static int tile_tz(int H) { return ((kRowLen - 4 * ((H + 3) / 4) - H) / 4) * 4; }
+// owned voxels of z per tile of the register-blocked kernel (RegShape::TZ)
+static int reg_tile_tz(int H) { return ((kRowLen - (H + (H & 1)) - H) / 2) * 2; }
 
 // Default workgroup shape.  32-row tiles (16 waves x 2) halve the y-halo
 // overhead of 16-row tiles where their LDS ring fits (one-byte codes,
@@ -1566,7 +1862,7 @@ struct BatchArgs {
 
 template <int H, int NW, int RPW>
 __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
-    using Sh = SweepShape<H, NW, RPW>;
+    using Sh = RegShape<H, NW, RPW>;
     const int per = gridDim.x >> 3;
     const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (logical >= a.n_tiles) return;
@@ -1582,14 +1878,14 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
     const int tx = t / nTy;
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
-    sweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, 0, v,
-                                           vol.omega, a.colour0, a.TX, tz, ty, tx);
+    rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
+                                            vol.omega, a.colour0, a.TX, tz, ty, tx);
 }
 
 template <int H, int NW, int RPW>
 static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
-    using Sh = SweepShape<H, NW, RPW>;
-    if constexpr (Sh::TY < 1) {
+    using Sh = RegShape<H, NW, RPW>;
+    if constexpr (Sh::TY < 2) {
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
     } else {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
@@ -1630,6 +1926,7 @@ static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {
         if (tile->tx > 0) t->tx = tile->tx;
     }
     if (t->tx < 1 || t->tx > 65536) return fail(-1, "invalid tile tx=%d", t->tx);
+    t->tx = (t->tx + 1) & ~1;  // the register-blocked kernel needs an even x tile
     if (t->waves * t->rows_per_wave <= 2 * n_half)
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", t->waves,
                     t->rows_per_wave, n_half);
@@ -1643,7 +1940,7 @@ extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_til
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
     const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
-    const long long tz = tile_tz(n_half);
+    const long long tz = reg_tile_tz(n_half);
     long long n = ((Z + tz - 1) / tz) * ((Y + ty - 1) / ty) * ((X + t.tx - 1) / t.tx);
     if (n >= (1LL << 28)) return fail(-1, "too many tiles");
     *count = (int)n;

@@ -101,10 +101,11 @@ def test_c5_crop24_voxel_gradient():
 
 def test_balanced_x_tile_at_512():
     """tx 0: the launch picks the x-tile length whose tiles fill the resident
-    workgroup slots in the fewest, shortest rounds.  A 16x2 workgroup at
-    n_half = 3 needs 123 KiB of LDS, so one is resident per CU; 512^3 has
-    5 x 20 tiles in z and y, and 5 x tiles of 103 planes (500 tiles, two
-    rounds of 256) beat 4 of 128 (400 tiles, also two rounds) and 2 of 256."""
+    workgroup slots in the fewest, shortest rounds.  A 16x2 workgroup of the
+    Solver sweep at n_half = 3 uses 106 VGPRs per lane, so one is resident
+    per CU; 512^3 has 5 x 20 tiles in z and y, and 5 x tiles of 104 planes
+    (500 tiles, two rounds of 256; the register-blocked kernel needs an even
+    x tile) beat 4 of 128 (400 tiles, also two rounds) and 2 of 256."""
     import ctypes
 
     from taufactor_amd import _lib
@@ -116,7 +117,7 @@ def test_balanced_x_tile_at_512():
     st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 0),
                                     ctypes.byref(out), ctypes.byref(n))
     assert st == 0, lib.tf_last_error()
-    assert (out.waves, out.rows_per_wave, out.tx, n.value) == (16, 2, 103, 500)
+    assert (out.waves, out.rows_per_wave, out.tx, n.value) == (16, 2, 104, 500)
     # an explicit x tile is kept
     st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 64),
                                     ctypes.byref(out), ctypes.byref(n))
