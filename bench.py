@@ -109,7 +109,7 @@ def main(argv=None) -> int:
     import torch
 
     import taufactor_amd as tau
-    from taufactor_amd import workloads
+    from taufactor_amd import autotune, workloads
 
     if args.config == "batch256":
         return bench_batch(args)
@@ -134,6 +134,8 @@ def main(argv=None) -> int:
     t0 = time.perf_counter()
     H, tile = s._launch_config()  # runs the autotuner unless both are given
     _, rtile, n_tiles = s.resolved_launch()
+    for ms, h, t in autotune.last_timings:
+        log(f"  autotune candidate H={h} tile {t}: {ms:.4f} ms per half-sweep")
     log(f"launch configuration: {H} half-sweeps per launch, tile {tile or 'library default'} "
         f"-> {rtile[0]}x{rtile[1]} waves x rows, x tile {rtile[2]}, {n_tiles} tiles "
         f"({'autotuned' if args.half_sweeps is None or args.tile is None else 'given'}, "

@@ -63,8 +63,9 @@ extern "C" {
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by TZ voxels in z
  * (TZ = 120 for n_half <= 4, 112 for 5 and 6) along tx planes of x; one
- * wavefront per row.  Compiled shapes (waves x rows_per_wave): 8x2, 4x4,
- * 12x2, 8x4, 16x2.  Zero fields select the defaults: 16x2 for one-byte
+ * wavefront per row.  Compiled shapes (waves x rows_per_wave): 8x2, 12x2
+ * and 16x2 at every n_half, 4x4 and 8x4 at n_half <= 4 (12x2 and 8x4 for
+ * the Solver kind and batches only).  Zero fields select the defaults: 16x2 for one-byte
  * codes with n_half <= 4 and Y >= 64, else 8x2 (8x2 for batches).  tx 0
  * (tf_sweep) lets the launch choose the x-tile length: the one whose tiles
  * fill the device's resident workgroup slots in the fewest, shortest

@@ -28,12 +28,14 @@ TIMED_LAUNCHES = 3
 # first candidate within TIE of the fastest wins, which keeps the choice
 # stable against timing noise.  x tile 0 is the library's occupancy-balanced
 # length for the shape and volume.
-SOLVER_SPACE = ((4, 3, 2), ((16, 2), (12, 2), (8, 2)), (0, 128, 64))
+SOLVER_SPACE = ((5, 4, 6, 3, 2), ((16, 2), (12, 2), (8, 2)), (0, 128, 64))
 FIELD_SPACE = ((4, 3, 2), ((8, 2), (4, 4)), (0, 128, 64))
 BATCH_SPACE = ((4, 2), ((8, 2), (4, 4), (12, 2), (16, 2)), (64, 32))
 TIE = 0.02
 
 _cache: dict = {}
+# (ms per half-sweep, H, tile) of every candidate of the latest tuning run
+last_timings: list = []
 
 
 def _candidates(space, fixed_h, fixed_tile):
@@ -94,6 +96,7 @@ def tune_solver(s, fixed_h=None, fixed_tile=None):
         if ms is not None:
             timings.append((ms / H, H, tile))
     del src, dst
+    last_timings[:] = timings
     result = _pick(timings)
     _cache[key] = result
     return result
@@ -128,6 +131,7 @@ def tune_batch(b, fixed_h=None, fixed_tile=None):
     finally:
         b._plan = None
         b.reset()
+    last_timings[:] = timings
     result = _pick(timings)
     _cache[key] = result
     return result

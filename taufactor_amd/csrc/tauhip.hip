@@ -590,6 +590,9 @@ struct SweepArgs {
     int colour0;
     int TX;
     int nTz, nTy, nTx, total;
+    // register-blocked kinds: tiles [0, nWide) are wide, the rest narrow
+    // (the last strip of z, from z0N, nTyN tiles of two y tiles each)
+    int nWide, nTyN, z0N;
     int resolve_only;  // set TX and the tile counts, launch nothing
 };
 
@@ -984,8 +987,11 @@ struct RegShape {
     static constexpr int TY = EY - 2 * H;
     static constexpr int ML = H + (H & 1);                    // z0 - ew: even, so ew is odd
     static constexpr int TZ = ((kRowLen - ML - H) / 2) * 2;   // owned voxels of z (even)
-    static constexpr int NR = ((H + 5) / 2) * 2;              // planes in registers (even)
-    static constexpr int D = NR - H - 2;                      // planes in flight (2 or 3)
+    static constexpr int TZN = ((kRowPairs - ML - H) / 2) * 2;  // the same for a narrow tile
+    // planes in registers (even): H + 2 in use and D in flight; D = 1 at
+    // H = 5 keeps a 16-wave workgroup within 128 VGPRs per lane
+    static constexpr int NR = H == 5 ? 8 : ((H + 5) / 2) * 2;
+    static constexpr int D = NR - H - 2;
     static constexpr int TBL = 640;                           // LDS bytes for the table
     static constexpr int EB = (NW + 2) * 2 * kRowPairs;       // floats per edge buffer
     static constexpr size_t lds_bytes() { return TBL + (size_t)H * 2 * EB * sizeof(float); }
@@ -1012,13 +1018,21 @@ __device__ __forceinline__ float from_next_lane(uint32_t v) {
 // Tile geometry requirements (checked by the launch): TX and TY even, so that
 // with the unclipped stream start S0 = x0 - (H-1) the active element of a
 // lane depends only on the launch's colour and the step and row parities.
+//
+// A narrow tile (nar) covers the last strip of z when it is at most TZN wide:
+// each half-wave holds a 64-voxel window of z, half 0 for the rows of y tile
+// 2*ty and half 1 for those of y tile 2*ty + 1 (TY rows further, an even
+// count, so the active element stays wave-uniform).  The DPP shifts mix the
+// halves only at lanes 31 and 32, which are halo lanes of both windows, and
+// the edge rows go through LDS per lane as before; only the addresses and
+// the stores (one per half) differ.
 template <int H, int KIND, int NW, int RPW>
 __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
-                                            int colour0, int TX, int tz, int ty, int tx) {
+                                            int colour0, int TX, int z0, int ty, int tx, bool nar) {
     using Sh = RegShape<H, NW, RPW>;
-    constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
+    constexpr int TY = Sh::TY, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
     static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
     static_assert(RPW % 2 == 0, "rows per wave must be even");
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1027,11 +1041,16 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
-    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
+    // y tile of each half (for a wide tile half 1 owns nothing)
+    const int y0h[2] = {1 + (nar ? 2 * ty : ty) * TY, 1 + (nar ? 2 * ty + 1 : ty) * TY};
+    const int x0 = 1 + tx * TX;
+    const int x1 = min(x0 + TX, v.X + 1);
+    const int z1 = min(z0 + (nar ? Sh::TZN : Sh::TZ), v.Z + 1);
+    const int half = nar ? lane >> 5 : 0;                 // this lane's half
+    const int y0 = half ? y0h[1] : y0h[0];                  // this lane's y tile
     const int ey0 = y0 - H;
-    const int ew = z0 - ML;        // odd
-    const int zl = ew + 2 * lane;  // z of the first voxel of this lane's pair
+    const int ew = z0 - ML;                                 // odd
+    const int zl = ew + 2 * (nar ? (lane & 31) : lane);     // z of the lane's first voxel
     const int lx1 = min(x1 + H, v.X + 2);
 
     if (KIND == TF_KIND_SOLVER) {
@@ -1055,16 +1074,20 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int plane_c = (v.Y + 2) * row_c;
     const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
     int vf[RPW], vc[RPW];
-    bool own_row[RPW];
-    int ys[RPW];
+    bool own_row[2][RPW];   // wave-uniform, per half
+    int ys[2][RPW];
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
         const int y = ey0 + wave * RPW + j;
         const bool rok = y >= 0 && y <= v.Y + 1;
         vf[j] = rok ? kSlackBytes + (y * v.sY + zl) * 4 : kOff;
         vc[j] = rok ? y * row_c + zl + 3 : kOff;
-        own_row[j] = y >= y0 && y < y1;
-        ys[j] = own_row[j] ? y : 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int yk = y0h[k] - H + wave * RPW + j;
+            own_row[k][j] = (k == 0 || nar) && yk >= y0h[k] && yk < min(y0h[k] + TY, v.Y + 1);
+            ys[k][j] = own_row[k][j] ? yk : 0;
+        }
     }
     u32x2_t F[NR][RPW];
     uint32_t K[NR][RPW];
@@ -1087,16 +1110,22 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
         }
     };
-    const int zs = zl >= z0 ? zl * 4 : kOff;
+    // store k writes the lanes of half k: one descriptor per row, plane and
+    // half, based at z = 0 of that row with record count 4*z1
+    const int zs0 = (zl >= z0 && half == 0) ? zl * 4 : kOff;
+    const int zs1 = (zl >= z0 && half == 1) ? zl * 4 : kOff;
     auto store_plane = [&](int q, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         const bool qok = q >= x0 && q < x1;
         const int qs = qok ? q : 0;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            const rsrc_t r = make_rsrc(D + qs * v.sX + ys[j] * v.sY,
-                                       (qok && own_row[j]) ? (uint32_t)(4 * z1) : 0u);
-            __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, zs, 0, 0);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const rsrc_t r = make_rsrc(D + qs * v.sX + ys[k][j] * v.sY,
+                                           (qok && own_row[k][j]) ? (uint32_t)(4 * z1) : 0u);
+                __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, k ? zs1 : zs0, 0, 0);
+            }
         }
     };
 
@@ -1245,10 +1274,17 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (logical >= a.total) return;
     int t = logical;
-    const int tz = t % a.nTz; t /= a.nTz;
-    const int ty = t % a.nTy; t /= a.nTy;
-    const int tx = t % a.nTx;
-    const int b = t / a.nTx;
+    const bool nar = logical >= a.nWide;
+    int tz = 0, ty, tx, b;
+    if (!nar) {
+        tz = t % a.nTz; t /= a.nTz;
+        ty = t % a.nTy; t /= a.nTy;
+    } else {
+        t -= a.nWide;
+        ty = t % a.nTyN; t /= a.nTyN;
+    }
+    tx = t % a.nTx;
+    b = t / a.nTx;
     const size_t off = (size_t)b * a.v.sB;
     const size_t coff = KIND == TF_KIND_FIELD ? off
                                               : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
@@ -1256,7 +1292,9 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     if constexpr (own_code_kind(KIND))
         rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
                                       static_cast<const uint8_t*>(a.codes) + coff, hm, a.v, a.w,
-                                      a.colour0, a.TX, tz, ty, tx);
+                                      a.colour0, a.TX,
+                                      nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx,
+                                      nar);
     else
         sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
                                      static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls,
@@ -1354,19 +1392,32 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
                 (void)hipGetLastError();
             attr_set = true;
         }
-        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
         a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
+        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
+        a.nTyN = 0;
+        if constexpr (Sh::reg) {
+            // wide tiles while a full one fits, then a narrow strip if the
+            // rest fits one (RegShape::TZN), else one more wide tile
+            const int tzn = RegShape<H, NW, RPW>::TZN;
+            const int wide = a.v.Z / Sh::TZ, rest = a.v.Z - wide * Sh::TZ;
+            a.nTz = wide + (rest > tzn ? 1 : 0);
+            if (rest > 0 && rest <= tzn) a.nTyN = (a.nTy + 1) / 2;
+            a.z0N = 1 + wide * Sh::TZ;
+        }
+        const long long yz = (long long)a.nTz * a.nTy + a.nTyN;
         if (a.TX <= 0) {
             static int slots_cache[kMaxDevices] = {};
             const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
                                                   lds, slots_cache);
-            a.TX = balanced_tx(a.v.X, (long long)a.nTz * a.nTy * B, H, Sh::NP, slots, Sh::reg);
+            a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, Sh::reg);
         }
         // the register-blocked kernel needs an even x tile (rsweep_tile)
         if (Sh::reg) a.TX = (a.TX + 1) & ~1;
         a.nTx = (a.v.X + a.TX - 1) / a.TX;
-        long long total = (long long)a.nTz * a.nTy * a.nTx * B;
+        const long long wide_tiles = (long long)a.nTz * a.nTy * a.nTx * B;
+        const long long total = yz * a.nTx * B;
         if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
+        a.nWide = (int)wide_tiles;
         a.total = (int)total;
         if (a.resolve_only) return 0;
         const int grid = (int)(((total + 7) / 8) * 8);
@@ -1382,13 +1433,18 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
 template <int H, int KIND>
 static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
     if (nw == 8 && rpw == 2) return launch_sweep<H, KIND, 8, 2>(a, B, st);
-    if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
     if (nw == 16 && rpw == 2) return launch_sweep<H, KIND, 16, 2>(a, B, st);
+    if constexpr (H <= 4) {
+        if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
+    }
     if constexpr (KIND == TF_KIND_SOLVER) {
         if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
-        if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
+        if constexpr (H <= 4) {
+            if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
+        }
     }
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for kind %d", nw, rpw, KIND);
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for kind %d and n_half %d",
+                nw, rpw, KIND, H);
 }
 
 template <int KIND>
@@ -1412,7 +1468,7 @@ static int reg_tile_tz(int H) { return ((kRowLen - (H + (H & 1)) - H) / 2) * 2; 
 // n_half <= 4) and the volume is tall enough; the x-tile length is left to
 // the launch (balanced_tx).  Any shape gives identical results.
 static void default_tile(int H, int kind, int Y, tf_tile* t) {
-    const bool tall = kind != TF_KIND_FIELD && H <= 4 && Y >= 64;
+    const bool tall = (own_code_kind(kind) || (kind != TF_KIND_FIELD && H <= 4)) && Y >= 64;
     t->waves = tall ? 16 : 8;
     t->rows_per_wave = 2;
     t->tx = 0;  // chosen at launch from the resident workgroups (balanced_tx)
@@ -1879,7 +1935,8 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
     rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
-                                            vol.omega, a.colour0, a.TX, tz, ty, tx);
+                                            vol.omega, a.colour0, a.TX, 1 + tz * Sh::TZ, ty, tx,
+                                            false);
 }
 
 template <int H, int NW, int RPW>
@@ -1908,11 +1965,13 @@ static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
 template <int H>
 static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
     if (nw == 8 && rpw == 2) return launch_sweep_batch<H, 8, 2>(a, st);
-    if (nw == 4 && rpw == 4) return launch_sweep_batch<H, 4, 4>(a, st);
     if (nw == 12 && rpw == 2) return launch_sweep_batch<H, 12, 2>(a, st);
-    if (nw == 8 && rpw == 4) return launch_sweep_batch<H, 8, 4>(a, st);
     if (nw == 16 && rpw == 2) return launch_sweep_batch<H, 16, 2>(a, st);
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled", nw, rpw);
+    if constexpr (H <= 4) {
+        if (nw == 4 && rpw == 4) return launch_sweep_batch<H, 4, 4>(a, st);
+        if (nw == 8 && rpw == 4) return launch_sweep_batch<H, 8, 4>(a, st);
+    }
+    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for n_half %d", nw, rpw, H);
 }
 
 static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {

@@ -103,9 +103,11 @@ def test_balanced_x_tile_at_512():
     """tx 0: the launch picks the x-tile length whose tiles fill the resident
     workgroup slots in the fewest, shortest rounds.  A 16x2 workgroup of the
     Solver sweep at n_half = 3 uses 106 VGPRs per lane, so one is resident
-    per CU; 512^3 has 5 x 20 tiles in z and y, and 5 x tiles of 104 planes
-    (500 tiles, two rounds of 256; the register-blocked kernel needs an even
-    x tile) beat 4 of 128 (400 tiles, also two rounds) and 2 of 256."""
+    per CU (256 slots, 32 per XCD).  512^3 has 20 y tiles of 26 rows and
+    four z tiles of 120 plus a narrow strip of 32 (10 narrow tiles of two y
+    tiles each): 90 tiles per x column.  Modelled cost, rounds x (even tx +
+    4 rounded up to 8 steps, + 4): 8 x tiles of 64 give 720 tiles, 3 rounds,
+    3 x 76 = 228; 5 of 104 give 2 x 116 = 232; 2 of 256 give 1 x 268."""
     import ctypes
 
     from taufactor_amd import _lib
@@ -117,8 +119,8 @@ def test_balanced_x_tile_at_512():
     st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 0),
                                     ctypes.byref(out), ctypes.byref(n))
     assert st == 0, lib.tf_last_error()
-    assert (out.waves, out.rows_per_wave, out.tx, n.value) == (16, 2, 104, 500)
+    assert (out.waves, out.rows_per_wave, out.tx, n.value) == (16, 2, 64, 720)
     # an explicit x tile is kept
-    st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 64),
+    st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 104),
                                     ctypes.byref(out), ctypes.byref(n))
-    assert st == 0 and out.tx == 64 and n.value == 800
+    assert st == 0 and out.tx == 104 and n.value == 450

@@ -47,15 +47,11 @@ def test_solver_iterates_bit_identical(shape, H):
     assert np.array_equal(_field(s), o.field.numpy())
 
 
-def _lds_bytes(tile, H, code_bytes=1):
-    return 512 + (H + 3) * tile[0] * tile[1] * 128 * (4 + code_bytes) + 72
-
-
 @pytest.mark.parametrize("tile", [(8, 2, 1), (4, 4, 9), (12, 2, 5), (8, 4, 2), (16, 2, 3)])
 @pytest.mark.parametrize("H", [2, 3, 4, 6])
 def test_solver_iterates_with_small_tiles(tile, H):
-    if _lds_bytes(tile, H) > 160 * 1024:
-        pytest.skip("shape needs more than 160 KiB of LDS at this depth")
+    if tile[1] == 4 and H > 4:
+        pytest.skip("four-row waves are compiled for n_half <= 4")
     img = _binary((19, 23, 300), seed=5)
     s = tau.Solver(img)
     s.half_sweeps_per_pass = H
