@@ -14,7 +14,9 @@ roofline    the sweep kernel (k_sweep): algorithmic bytes per launch =
             1 B code: one unblocked pass per voxel-update) x the voxel-updates
             one launch performs (voxels x H/2 for H fused half-sweeps),
             divided by the kernel's mean duration measured with HIP events on
-            its stream during the timed steps; peak 8 TB/s.  Temporal
+            its stream during the timed steps, over the launches that fuse
+            all H half-sweeps (with 100 % H != 0 each check interval ends
+            with a shorter launch, another kernel, left out); peak 8 TB/s.  Temporal
             blocking lets this exceed the bytes actually moved: `traffic`
             holds the measured HBM bytes per launch (rocprofv3 FETCH_SIZE +
             WRITE_SIZE, calibrated), when a summary for this H and tile exists
@@ -164,12 +166,15 @@ def main(argv=None) -> int:
     launches = sum(n for _, _, n in events)
     nvox = img.size
     per_launch_s = sweep_ms / 1e3 / launches
-    # every launch of the timed solves fuses H half-sweeps unless 100 % H != 0
-    updates_per_launch = nvox * sum(iters) / 2 / launches
+    # the timed launches all fuse H half-sweeps (the solver leaves a shorter
+    # last launch of a check interval out of its events): H/2 voxel-updates
+    # per voxel per launch
+    updates_per_launch = nvox * H / 2
     achieved = BYTES_PER_VOXEL_PASS * updates_per_launch / per_launch_s / 1e9
     gvus = nvox * sum(iters) / 2 / elapsed / 1e9
     log(f"{args.steps} solves, iterations {iters}, {elapsed:.3f} s; sweep kernel mean "
-        f"{per_launch_s * 1e3:.3f} ms over {launches} launches -> {achieved:.0f} GB/s")
+        f"{per_launch_s * 1e3:.3f} ms over {launches} launches of k_sweep<{H}, ...> -> "
+        f"{achieved:.0f} GB/s")
 
     out = {
         "metric": METRIC,
@@ -213,12 +218,14 @@ def main(argv=None) -> int:
     traffic_file = os.path.join(ROOT, "profiles", f"k_sweep_solver_{args.size}_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            t = json.load(f)
-        if t.get("half_sweeps_per_pass") == H and t.get("tile") == list(rtile):
-            out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
-            out["roofline"]["traffic_source"] = ("rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch, "
-                                                 "profiles/k_sweep_solver_%d_traffic.md"
-                                                 % (t["fetch_factor"], t["write_factor"], args.size))
+            entries = json.load(f)
+        for t in entries if isinstance(entries, list) else [entries]:
+            if t.get("half_sweeps_per_pass") == H and t.get("tile") == list(rtile):
+                out["roofline"]["traffic"] = t["hbm_bytes_per_launch"]
+                out["roofline"]["traffic_source"] = (
+                    "rocprofv3 FETCH_SIZE x %.1f + WRITE_SIZE x %.1f per launch of %s, "
+                    "profiles/k_sweep_solver_%d_traffic.md"
+                    % (t["fetch_factor"], t["write_factor"], t["kernel"], args.size))
     if not args.no_cpu_baseline:
         log(f"cpu baseline: {args.cpu_iters} oracle iterations at {args.size}^3 ...")
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_iters)

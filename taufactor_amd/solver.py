@@ -219,12 +219,16 @@ class SORSolver(ABC):
         codes = _lib.ptr(self._sweep_codes if self._sweep_codes is not None else self._codes)
         hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
         it = self.iter
+        # timing (bench.py): HIP events on the launch stream around the
+        # launches that fuse the full `per` half-sweeps (a shorter last
+        # launch is another kernel and is left out)
         ev = self._launch_events
-        if ev is not None:
+        full = n_half // per
+        if ev is not None and full:
             t0 = torch.cuda.Event(enable_timing=True)
             t1 = torch.cuda.Event(enable_timing=True)
             t0.record()
-            launches = 0
+        launches = 0
         while n_half > 0:
             h = min(per, n_half)
             src = self._fields[self._cur]
@@ -236,11 +240,10 @@ class SORSolver(ABC):
             self._cur = 1 - self._cur
             it += h
             n_half -= h
-            if ev is not None:
-                launches += 1
-        if ev is not None:
-            t1.record()
-            ev.append((t0, t1, launches))
+            launches += 1
+            if ev is not None and launches == full:
+                t1.record()
+                ev.append((t0, t1, full))
 
     def _layer_sums(self):
         """Per-layer float64 flux and concentration sums, copied to the host."""
