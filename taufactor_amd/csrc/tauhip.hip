@@ -7,27 +7,17 @@
 // reference value for value.  Fused multiply-adds appear only where written
 // explicitly (the exact-division sequence in div_by_count).
 //
-// Layout and conventions are documented in include/tauhip.h.
+// Layout and conventions are documented in include/tauhip.h.  The sweep
+// kernels live in tauhip_sweep.h and are instantiated in sweep_<kind>.hip.
 
-#include <hip/hip_runtime.h>
-
-#include <stdarg.h>
-#include <stdint.h>
-#include <stdio.h>
-#include <string.h>
-
-#include <algorithm>
-#include <type_traits>
-#include <vector>
-
-#include "tauhip.h"
+#include "tauhip_sweep.h"
 
 // ---------------------------------------------------------------------------
 // error plumbing
 // ---------------------------------------------------------------------------
 static thread_local char g_err[512] = "";
 
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
@@ -35,99 +25,13 @@ static int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-static int hip_status(hipError_t e, const char* what) {
+int hip_status(hipError_t e, const char* what) {
     if (e == hipSuccess) return 0;
     return fail((int)e, "%s: %s", what, hipGetErrorString(e));
 }
 
-#define TF_LAUNCH_CHECK(what)                                   \
-    do {                                                        \
-        hipError_t e_ = hipGetLastError();                      \
-        if (e_ != hipSuccess) return hip_status(e_, what);      \
-    } while (0)
-
 extern "C" int tf_abi_version(void) { return TF_ABI_VERSION; }
 extern "C" const char* tf_last_error(void) { return g_err; }
-
-// ---------------------------------------------------------------------------
-// geometry of one padded volume
-// ---------------------------------------------------------------------------
-struct Vol {
-    int X, Y, Z;     // interior extents
-    int sY, sX;      // padded strides: sY = Z+2, sX = (Y+2)(Z+2)
-    long long sB;    // padded volume size (X+2)(Y+2)(Z+2)
-};
-
-__host__ __device__ static inline Vol make_vol(int X, int Y, int Z) {
-    Vol v;
-    v.X = X; v.Y = Y; v.Z = Z;
-    v.sY = Z + 2;
-    v.sX = (Y + 2) * (Z + 2);
-    v.sB = (long long)(X + 2) * v.sX;
-    return v;
-}
-
-// sweep-code layout: rows of sweep_code_row(Z) bytes (a multiple of 4),
-// byte z+3 of a row = code of padded voxel z (z in [0, Z+1]), other bytes 0
-__host__ __device__ inline int sweep_code_row(int Z) { return ((Z + 5 + 3) / 4) * 4; }
-__host__ __device__ inline long long sweep_code_volume(int X, int Y, int Z) {
-    return (long long)(X + 2) * (Y + 2) * sweep_code_row(Z);
-}
-
-static int check_dims(int B, int X, int Y, int Z) {
-    if (B < 1 || X < 1 || Y < 1 || Z < 1)
-        return fail(-1, "invalid volume shape B=%d X=%d Y=%d Z=%d", B, X, Y, Z);
-    // offsets inside one padded volume are 32-bit
-    long long n = (long long)(X + 2) * (Y + 2) * (Z + 2);
-    if (n >= (1LL << 31))
-        return fail(-1, "padded volume of %lld voxels exceeds 2^31", n);
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
-// raw buffer access: 32-bit offsets from a uniform base; an offset at or past
-// the record count reads 0 and drops a write
-// ---------------------------------------------------------------------------
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer descriptor on gfx9
-
-__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kBufferFlags);
-}
-// byte_off: per-lane offset (range-checked against the record count);
-// soff: wave-uniform offset added to the address
-__device__ __forceinline__ float ld_f32(rsrc_t r, int byte_off, int soff = 0) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, soff, 0));
-}
-__device__ __forceinline__ uint32_t ld_u8(rsrc_t r, int byte_off, int soff = 0) {
-    return __builtin_amdgcn_raw_buffer_load_b8(r, byte_off, soff, 0);
-}
-__device__ __forceinline__ void st_f32(float val, rsrc_t r, int byte_off, int soff = 0) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val), r, byte_off, soff, 0);
-}
-
-// ---------------------------------------------------------------------------
-// exact float32 division by a conductive-neighbour count
-// ---------------------------------------------------------------------------
-// q = s / d for d in {inf, 1..8}, correctly rounded (the value IEEE division
-// gives).  One reciprocal product plus one residual correction with the
-// correctly rounded reciprocal r = RN(1/d); numerators below 2^-100 (where
-// the correction can be off for d = 6, and where the sign of zero matters)
-// take the IEEE division.  Exhaustively checked against s/d over all finite
-// float32 s for each d (tests/test_division.py samples that check).
-__device__ __forceinline__ float div_by_count(float s, float r, float d, int code) {
-    float q = s * r;
-    if (code != 0) {
-        float e = __builtin_fmaf(-q, d, s);
-        q = __builtin_fmaf(e, r, q);
-    }
-    if (__builtin_expect(__builtin_fabsf(s) < 0x1p-100f, 0)) q = s / d;
-    return q;
-}
-
-// table of (1/d, d) per Solver code; code 0 means factor = inf
-__constant__ float c_rcp[9] = {0.0f, 1.0f, 0.5f, 1.0f / 3.0f, 0.25f,
-                               0.2f, 1.0f / 6.0f, 1.0f / 7.0f, 0.125f};
 
 // ---------------------------------------------------------------------------
 // set-up kernels
@@ -435,56 +339,6 @@ extern "C" int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, i
 }
 
 // ---------------------------------------------------------------------------
-// the update of one voxel (shared by both sweep kernels)
-// ---------------------------------------------------------------------------
-// Solver (taufactor/taufactor.py:97-102, 177-181):
-//   s = ((((c[x+1]+c[x-1])+c[y+1])+c[y-1])+c[z+1])+c[z-1]
-//   s = s / nn ; s = s - c ; s = s * w ; c = c + s
-// MultiPhase (taufactor/taufactor.py:598-613):
-//   s = c[x+1]*gx+ + c[x-1]*gx- + c[y+1]*gy+ + c[y-1]*gy- + c[z+1]*gz+ + c[z-1]*gz-
-//   f = gx- + gx+ + gy- + gy+ + gz- + gz+ ; f += gx- at x=0 ; f += gx+ at x=X-1 ;
-//   f = inf if f == 0 ;  s = s / f ; s = s - c ; s = s * w ; c = c + s
-struct Nbr {
-    float c, xp, xm, yp, ym, zp, zm;
-};
-
-__device__ __forceinline__ float update_solver(const Nbr& n, int code, float w) {
-    float s = n.xp + n.xm;
-    s = s + n.yp;
-    s = s + n.ym;
-    s = s + n.zp;
-    s = s + n.zm;
-    float r = c_rcp[code];
-    float d = code ? (float)code : __builtin_inff();
-    s = div_by_count(s, r, d, code);
-    s = s - n.c;
-    s = s * w;
-    return n.c + s;
-}
-
-__device__ __forceinline__ float update_multiphase(const Nbr& n, float gxp, float gxm, float gyp,
-                                                   float gym, float gzp, float gzm, bool xfirst,
-                                                   bool xlast, float w) {
-    float s = n.xp * gxp + n.xm * gxm;
-    s = s + n.yp * gyp;
-    s = s + n.ym * gym;
-    s = s + n.zp * gzp;
-    s = s + n.zm * gzm;
-    float f = gxm + gxp;
-    f = f + gym;
-    f = f + gyp;
-    f = f + gzm;
-    f = f + gzp;
-    if (xfirst) f = f + gxm;
-    if (xlast) f = f + gxp;
-    if (f == 0.0f) f = __builtin_inff();
-    s = s / f;
-    s = s - n.c;
-    s = s * w;
-    return n.c + s;
-}
-
-// ---------------------------------------------------------------------------
 // unblocked in-place half-sweep: one thread per interior voxel
 // ---------------------------------------------------------------------------
 template <int KIND>
@@ -543,923 +397,6 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
     return 0;
 }
 
-// ---------------------------------------------------------------------------
-// temporally blocked sweep: H half-sweeps per HBM pass
-// ---------------------------------------------------------------------------
-// A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
-// coordinates) and streams it plane by plane along x through a ring of H+3
-// planes in LDS, each plane extended by a halo in y and z.  Half-sweep h
-// (h = 0..H-1, colour first_colour^(h&1)) runs on plane s-h at step s, on
-// the tile grown by H-1-h voxels, so that after the last half-sweep the
-// owned tile holds exactly the values of H successive reference iterations.
-// Red-black colouring makes each half-sweep a Jacobi step within one colour,
-// so reading src and writing dst (no in-place hazard between workgroups
-// that recompute each other's halos) gives the in-place result.
-//
-// Compute mapping: one wavefront per tile row (y), one lane per pair of
-// voxels in z, so an LDS row is a window of 128 voxels of z starting at ew
-// and the colour pattern of a row is uniform across the wave.  An LDS row
-// stores the first voxel of each pair (64), then the second (64): every LDS
-// access of a half-sweep is lane-contiguous (no bank conflicts).  Each wave
-// owns RPW rows (interleaved, NW even so that all rows of a wave share their
-// colour parity), NW waves make EY = NW*RPW rows (tile interior TY = EY-2H).
-// The window starts ML >= H voxels before the tile, with ML and the tile
-// width TZ multiples of 4 so that z0 and ew are 1 mod 4.
-//
-// Memory mapping: a plane of the tile is read with one 16-byte load per lane
-// covering two rows (lanes 0-31: row 2m, lanes 32-63: row 2m+1, four z
-// each), codes with one 4-byte load per lane from the aligned sweep-code
-// layout, and written back with one 8-byte store per lane and row (one pair
-// each).  The x loop is unrolled by the ring length NP, and plane q lives in
-// LDS slot and register slot (q - S0 + 1) mod NP: every slot is a
-// compile-time constant, so each LDS access is one instruction with an
-// immediate offset from one of a few per-lane base addresses.  Planes are
-// loaded NP-1 steps ahead of their use.
-//
-// Workgroups are numbered so that consecutive tiles land on the same XCD
-// (hardware dispatch sends workgroup i to XCD i mod 8): neighbouring tiles
-// then share their halo lines in that XCD's L2.
-struct SweepArgs {
-    const float* src;
-    float* dst;
-    const void* codes;
-    const float* hm;
-    int ncls;
-    Vol v;
-    float w;
-    int colour0;
-    int TX;
-    int nTz, nTy, nTx, total;
-    // register-blocked kinds: tiles [0, nWide) are wide, the rest narrow
-    // (the last strip of z, from z0N, nTyN tiles of two y tiles each)
-    int nWide, nTyN, z0N;
-    int resolve_only;  // set TX and the tile counts, launch nothing
-};
-
-// compile-time loop: f(std::integral_constant<int, I>{}) for I in [B, E)
-template <int B, int E, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-template <int I>
-using ic = std::integral_constant<int, I>;
-
-typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
-
-constexpr int kRowPairs = 64;           // lanes per row
-constexpr int kRowLen = 2 * kRowPairs;  // voxels per LDS row
-constexpr int kLdsPad = kRowLen * 4;    // bytes before the plane ring: keeps the row-below
-                                        // bias of the base addresses non-negative
-constexpr int kSlackBytes = TF_FIELD_SLACK_BYTES;
-
-template <int H, int NW, int RPW>
-struct SweepShape {
-    static constexpr int EY = NW * RPW;
-    static constexpr int TY = EY - 2 * H;
-    static constexpr int ML = 4 * ((H + 3) / 4);                // z0 - ew
-    static constexpr int TZ = ((kRowLen - ML - H) / 4) * 4;     // owned voxels of z
-    static constexpr int NP = H + 3;         // planes in the LDS ring and in the register ring
-    static constexpr int PS = EY * kRowLen;  // floats (and codes) per plane
-    static constexpr int RPL = RPW / 2;      // row pairs loaded per wave and plane
-};
-
-// per-voxel auxiliary data: a one-byte code (Solver, MultiPhase) or the
-// voxel diffusivity (per-voxel field)
-template <int KIND>
-using code_t = std::conditional_t<KIND == TF_KIND_FIELD, float, uint8_t>;
-
-// 2ab/(a+b), 0 when a+b == 0, with the reference's float32 operation order
-// ((2*a)*b)/(a+b) (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
-__device__ __forceinline__ float hmean(float a, float b) {
-    const float den = a + b;
-    return den > 0.0f ? ((2.0f * a) * b) / den : 0.0f;
-}
-
-// (1/d, d) per Solver code for the sweep.  Code 0 (factor inf) uses (0, 0):
-// the sequence of div_by_count then yields a zero whose sign can differ from
-// s/inf only when s is -0, and c + w*(q - c) is the same float for either
-// zero q whatever c is (c = +-0 gives +0, c != 0 gives c - w*c).
-__constant__ float c_rd[18] = {0.0f, 0.0f, 1.0f, 1.0f, 0.5f, 2.0f, 1.0f / 3.0f, 3.0f, 0.25f, 4.0f,
-                               0.2f, 5.0f, 1.0f / 6.0f, 6.0f, 1.0f / 7.0f, 7.0f, 0.125f, 8.0f};
-
-template <int H, int KIND, int NW, int RPW>
-__device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
-                                           const code_t<KIND>* __restrict__ Cg,
-                                           const float* __restrict__ hm, int ncls, const Vol v,
-                                           float w, int colour0, int TX, int tz, int ty, int tx) {
-    using Sh = SweepShape<H, NW, RPW>;
-    using CT = code_t<KIND>;
-    constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NP = Sh::NP, PS = Sh::PS, RPL = Sh::RPL;
-    static_assert(TY >= 1, "tile has no interior rows");
-    static_assert(NW % 2 == 0, "the rows of a wave must share their colour parity");
-    static_assert(RPW % 2 == 0, "rows are loaded in pairs");
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* sF = reinterpret_cast<float*>(smem + kLdsPad);
-    CT* sC = reinterpret_cast<CT*>(sF + NP * PS);
-    float* sT = reinterpret_cast<float*>(sC + NP * PS);
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
-    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
-    const int ey0 = y0 - H;
-    const int ew = z0 - ML;         // z of window element 0; ew = 1 mod 4
-    const int zl = ew + 2 * lane;   // z of the first voxel of this lane's pair
-    const int lx1 = min(x1 + H, v.X + 2);
-
-    if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
-    } else if (KIND == TF_KIND_MULTIPHASE) {
-        for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
-    } else if (KIND == TF_KIND_ANISOTROPIC) {
-        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    } else if (KIND == TF_KIND_ELECTRODE) {
-        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    }
-    // spacing weights of TF_KIND_ANISOTROPIC (wave-uniform)
-    const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
-    const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
-
-    // ---- global memory ----
-    // Loads: one descriptor per array for the whole tile.  The field (and
-    // the per-voxel diffusivity) descriptor starts kSlackBytes before the
-    // volume and spans it plus the slack on both sides, so the window's
-    // first quad (z = ew >= -3) is never at a negative offset; the lane
-    // offset holds row and z, the wave-uniform offset the plane.  Rows
-    // outside [0, Y+1] and planes past the tile's last plane use a lane
-    // offset past any record count: loads read 0, no memory traffic.  The
-    // range check is per dword, so quads running past the volume read 0
-    // there.
-    // Stores: one descriptor per row and plane, based at z = 0 of that row
-    // with record count 4*z1 (0 outside the owned rows and planes), so a
-    // pair straddling z1 = Z+1 stores only its first voxel; pairs before z0
-    // use the out-of-range lane offset.
-    // No branches around memory instructions, so the waitcnt pass keeps NP-1
-    // planes in flight.
-    constexpr int kOff = 0x7fffff00;  // lane offset past any record count
-    const int plane_f = v.sX * 4;
-    const int vol_f = (int)(v.sB * 4);
-    const int half = lane >> 5, qd = lane & 31;
-    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
-                                (uint32_t)vol_f + 2u * kSlackBytes);
-    int vf[RPL], vc[RPL];
-    rsrc_t rC;
-    int plane_c;
-    if constexpr (KIND == TF_KIND_FIELD) {
-        plane_c = plane_f;
-        rC = make_rsrc(reinterpret_cast<const char*>(Cg) - kSlackBytes,
-                       (uint32_t)vol_f + 2u * kSlackBytes);
-    } else {
-        const int row_c = sweep_code_row(v.Z);
-        plane_c = (v.Y + 2) * row_c;
-        rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
-    }
-#pragma unroll
-    for (int i = 0; i < RPL; ++i) {
-        const int y = ey0 + 2 * (wave + i * NW) + half;
-        const bool rok = y >= 0 && y <= v.Y + 1;
-        vf[i] = rok ? kSlackBytes + (y * v.sY + ew + 4 * qd) * 4 : kOff;
-        if constexpr (KIND == TF_KIND_FIELD)
-            vc[i] = vf[i];
-        else
-            vc[i] = rok ? y * sweep_code_row(v.Z) + (ew + 3) + 4 * qd : kOff;
-    }
-
-    u32x4_t pf[NP][RPL];
-    std::conditional_t<KIND == TF_KIND_FIELD, u32x4_t, uint32_t> pc[NP][RPL];
-    auto issue_load = [&](int p, auto reg_c) {
-        constexpr int rg = decltype(reg_c)::value;
-        const bool pok = p < lx1;
-        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
-#pragma unroll
-        for (int i = 0; i < RPL; ++i) {
-            pf[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rF, pok ? vf[i] : kOff, sf, 0);
-            if constexpr (KIND == TF_KIND_FIELD)
-                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rC, pok ? vc[i] : kOff, sc, 0);
-            else
-                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b32(rC, pok ? vc[i] : kOff, sc, 0);
-        }
-    };
-    // this lane's first quad slot (pairs 2qd, 2qd+1) in row 2*wave+half of slot 0
-    float* const Fl = sF + (2 * wave + half) * kRowLen + 2 * qd;
-    CT* const Cl = sC + (2 * wave + half) * kRowLen + 2 * qd;
-    auto commit_load = [&](auto slot_c, auto reg_c) {
-        constexpr int sl = decltype(slot_c)::value, rg = decltype(reg_c)::value;
-#pragma unroll
-        for (int i = 0; i < RPL; ++i) {
-            const int o = sl * PS + 2 * i * NW * kRowLen;
-            const u32x4_t q = pf[rg][i];
-            *reinterpret_cast<u32x2_t*>(Fl + o) = u32x2_t{q.x, q.z};               // first voxels
-            *reinterpret_cast<u32x2_t*>(Fl + o + kRowPairs) = u32x2_t{q.y, q.w};   // second voxels
-            if constexpr (KIND == TF_KIND_FIELD) {
-                const u32x4_t k = pc[rg][i];
-                *reinterpret_cast<u32x2_t*>(Cl + o) = u32x2_t{k.x, k.z};
-                *reinterpret_cast<u32x2_t*>(Cl + o + kRowPairs) = u32x2_t{k.y, k.w};
-            } else {
-                const uint32_t c = pc[rg][i];  // codes of z = 4qd+ew .. +3, byte 0 first
-                *reinterpret_cast<uint16_t*>(Cl + o) =
-                    (uint16_t)((c & 0xffu) | ((c >> 8) & 0xff00u));
-                *reinterpret_cast<uint16_t*>(Cl + o + kRowPairs) =
-                    (uint16_t)(((c >> 8) & 0xffu) | ((c >> 16) & 0xff00u));
-            }
-        }
-    };
-    // stores: rows of the compute mapping, one pair per lane
-    const float* Fw = sF + wave * kRowLen + lane;
-    const int zs = zl >= z0 ? zl * 4 : kOff;
-    bool own_row[RPW];
-    int ys[RPW];
-#pragma unroll
-    for (int j = 0; j < RPW; ++j) {
-        const int y = ey0 + wave + j * NW;
-        own_row[j] = y >= y0 && y < y1;
-        ys[j] = own_row[j] ? y : 0;
-    }
-    auto store_plane = [&](int q, auto slot_c) {
-        constexpr int sl = decltype(slot_c)::value;
-        const bool qok = q >= x0 && q < x1;
-        const int qs = qok ? q : 0;
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            const int o = sl * PS + j * NW * kRowLen;
-            const rsrc_t r = make_rsrc(D + qs * v.sX + ys[j] * v.sY,
-                                       (qok && own_row[j]) ? (uint32_t)(4 * z1) : 0u);
-            const u32x2_t val{__builtin_bit_cast(uint32_t, Fw[o]),
-                              __builtin_bit_cast(uint32_t, Fw[o + kRowPairs])};
-            __builtin_amdgcn_raw_buffer_store_b64(val, r, zs, 0, 0);
-        }
-    };
-
-    // ---- per-tile bounds of each half-sweep ----
-    // planes [plo[h], plo[h] + pn[h]) and, as bit h*RPW+j of rowbits, the
-    // active rows.  In z every lane updates its active element: outside the
-    // half-sweep's region the result is never read by a later half-sweep of
-    // this pass nor stored, and on the z ghosts (value 0, code 0 / class 0 /
-    // k 0) the update returns the ghost's value.
-    int plo[H], pn[H];
-    int rowbits = 0;
-#pragma unroll
-    for (int h = 0; h < H; ++h) {
-        const int grow = H - 1 - h;
-        plo[h] = max(x0 - grow, 1);
-        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
-        const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            const int y = ey0 + wave + j * NW;
-            rowbits |= (y >= ry0 && y < ry1) ? 1 << (h * RPW + j) : 0;
-        }
-    }
-
-    // The colour of (p, y, z) is (p+y+z+1)&1 and half-sweep h of step s
-    // updates colour colour0^(h&1) on plane s-h, so the active element of a
-    // lane's pair, act = (s + y + ew + 1 + colour0) & 1, is the same for all
-    // half-sweeps of a step and all rows of a wave; it alternates with s.
-    const int S0 = max(x0 - (H - 1), 1);
-    const int S1 = (x1 - 1) + (H - 1);
-    int A = (S0 + ey0 + wave + ew + 1 + colour0) & 1;  // act of the even steps of an iteration
-
-    // base addresses of the own element and its z neighbours for either
-    // parity of the step, biased one row down (the row below is at offset 0)
-    const float* Fo[2];
-    const float* Fp[2];
-    const float* Fm[2];
-    const CT* Co[2];
-    const CT* Cp[2];
-    const CT* Cm[2];
-    auto set_bases = [&](int a0) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int a = a0 ^ q;
-            const int r0 = wave * kRowLen - kRowLen + lane;
-            const int io = r0 + (a ? kRowPairs : 0);
-            const int ip = r0 + (a ? 1 : kRowPairs);      // z+1: next pair's first / own second
-            const int im = r0 + (a ? 0 : kRowPairs - 1);  // z-1: own first / previous pair's second
-            Fo[q] = sF + io; Fp[q] = sF + ip; Fm[q] = sF + im;
-            Co[q] = sC + io; Cp[q] = sC + ip; Cm[q] = sC + im;
-        }
-    };
-
-    // ---- one half-sweep of one plane ----
-    auto stage = [&](auto h_c, auto q_c, auto sp_c, int p) {
-        constexpr int h = decltype(h_c)::value, q = decltype(q_c)::value;
-        constexpr int sp = decltype(sp_c)::value;
-        constexpr int spm = (sp + NP - 1) % NP, spp = (sp + 1) % NP;
-        if ((unsigned)(p - plo[h]) >= (unsigned)pn[h]) return;
-        float* const Fq = const_cast<float*>(Fo[q]);
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            if (!((rowbits >> (h * RPW + j)) & 1)) continue;
-            const int o = sp * PS + j * NW * kRowLen + kRowLen;
-            const int om = spm * PS + j * NW * kRowLen + kRowLen;
-            const int op = spp * PS + j * NW * kRowLen + kRowLen;
-            Nbr n;
-            n.c = Fq[o];
-            n.xm = Fq[om];
-            n.xp = Fq[op];
-            n.ym = Fq[o - kRowLen];
-            n.yp = Fq[o + kRowLen];
-            n.zp = Fp[q][o];
-            n.zm = Fm[q][o];
-            float out;
-            if constexpr (KIND == TF_KIND_SOLVER) {
-                const int code = Co[q][o];
-                const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
-                float s = n.xp + n.xm;
-                s = s + n.yp;
-                s = s + n.ym;
-                s = s + n.zp;
-                s = s + n.zm;
-                float qt = s * rdv.x;
-                const float e = __builtin_fmaf(-qt, rdv.y, s);
-                qt = __builtin_fmaf(e, rdv.x, qt);
-                // below 2^-100 the residual correction can be off (d = 6):
-                // IEEE division there (s = +-0 needs none: any zero quotient
-                // gives the same update)
-                if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
-                    qt = s / (code ? (float)code : __builtin_inff());
-                s = qt - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_ELECTRODE) {
-                // SORSolver.sum_weighted_neighbours / factor with the reaction
-                // term in the factor (taufactor/taufactor.py:47-56, 95-103, 175-181)
-                float s = n.xp + n.xm;
-                s = s + n.yp;
-                s = s + n.ym;
-                s = s + n.zp;
-                s = s + n.zm;
-                s = s / sT[Co[q][o]];
-                s = s - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
-                // ((x+ + x-) + Ky*(y+ + y-)) + Kz*(z+ + z-), / nn
-                // (AnisotropicSolver.sum_weighted_neighbours, taufactor/taufactor.py:473-478)
-                float s = n.xp + n.xm;
-                float t = n.yp + n.ym;
-                t = ky * t;
-                s = s + t;
-                t = n.zp + n.zm;
-                t = kz * t;
-                s = s + t;
-                s = s / sT[Co[q][o]];
-                s = s - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
-                const float* row = sT + (int)Co[q][o] * ncls;
-                out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]], row[Co[q][o + kRowLen]],
-                                        row[Co[q][o - kRowLen]], row[Cp[q][o]], row[Cm[q][o]],
-                                        p == 1, p == v.X, w);
-            } else {
-                const float kc = Co[q][o];
-                out = update_multiphase(n, hmean(kc, Co[q][op]), hmean(kc, Co[q][om]),
-                                        hmean(kc, Co[q][o + kRowLen]), hmean(kc, Co[q][o - kRowLen]),
-                                        hmean(kc, Cp[q][o]), hmean(kc, Cm[q][o]), p == 1, p == v.X,
-                                        w);
-            }
-            Fq[o] = out;
-        }
-    };
-
-    // planes S0-1 .. S0+1 go straight to LDS (slots 0..2); S0+2 .. S0+NP are put in flight
-    issue_load(S0 - 1, ic<0>{});
-    commit_load(ic<0>{}, ic<0>{});
-    issue_load(S0, ic<1>{});
-    commit_load(ic<1>{}, ic<1>{});
-    issue_load(S0 + 1, ic<2>{});
-    commit_load(ic<2>{}, ic<2>{});
-    static_for<2, NP + 1>([&](auto d_c) {
-        constexpr int d = decltype(d_c)::value;
-        issue_load(S0 + d, ic<(d + 1) % NP>{});
-    });
-    __syncthreads();
-
-    // step s = s0 + u: put plane s+1+NP in flight, run half-sweep h on plane
-    // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS.
-    // Steps past S1 (to complete the last iteration) do no work but keep the
-    // memory instructions unconditional.
-    for (int s0 = S0; s0 <= S1; s0 += NP) {
-        set_bases(A);
-        static_for<0, NP>([&](auto u_c) {
-            constexpr int u = decltype(u_c)::value;
-            const int s = s0 + u;
-            issue_load(s + 1 + NP, ic<(u + 2) % NP>{});
-            static_for<0, H>([&](auto h_c) {
-                constexpr int h = decltype(h_c)::value;
-                stage(h_c, ic<(u & 1)>{}, ic<((u + 1 - h) % NP + NP) % NP>{}, s - h);
-                if constexpr (h + 1 < H) __syncthreads();
-            });
-            store_plane(s - (H - 1), ic<((u + 2 - H) % NP + NP) % NP>{});
-            commit_load(ic<(u + 3) % NP>{}, ic<(u + 3) % NP>{});
-            __syncthreads();
-        });
-        A ^= NP & 1;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// register-blocked sweep for the kinds whose update reads only the voxel's
-// own code (Solver, anisotropic, electrode)
-// ---------------------------------------------------------------------------
-// Same compute mapping as sweep_tile (a lane holds the z-pair zl, zl+1 of
-// each of its rows; the red/black pattern is wave-uniform), but a wave owns
-// RPW consecutive rows and keeps every plane of the ring in registers:
-// x neighbours and the y neighbours inside the wave are registers, the z
-// neighbour across lanes comes by a DPP wave shift, and only the first and
-// last row of each wave go through LDS.  Half-sweep h of step s reads the
-// edge rows that the neighbouring waves published at step s-1 (the values
-// of plane s-h after half-sweep h-1, or as loaded for h = 0), double
-// buffered by step parity, so one barrier per step suffices.
-__host__ __device__ constexpr bool own_code_kind(int kind) {
-    return kind == TF_KIND_SOLVER || kind == TF_KIND_ANISOTROPIC || kind == TF_KIND_ELECTRODE;
-}
-
-template <int H, int NW, int RPW>
-struct RegShape {
-    static constexpr int EY = NW * RPW;
-    static constexpr int TY = EY - 2 * H;
-    static constexpr int ML = H + (H & 1);                    // z0 - ew: even, so ew is odd
-    static constexpr int TZ = ((kRowLen - ML - H) / 2) * 2;   // owned voxels of z (even)
-    static constexpr int TZN = ((kRowPairs - ML - H) / 2) * 2;  // the same for a narrow tile
-    // planes in registers (even): H + 2 in use and D in flight; D = 1 at
-    // H = 5 keeps a 16-wave workgroup within 128 VGPRs per lane
-    static constexpr int NR = H == 5 ? 8 : ((H + 5) / 2) * 2;
-    static constexpr int D = NR - H - 2;
-    static constexpr int TBL = 640;                           // LDS bytes for the table
-    static constexpr int EB = (NW + 2) * 2 * kRowPairs;       // floats per edge buffer
-    static constexpr size_t lds_bytes() { return TBL + (size_t)H * 2 * EB * sizeof(float); }
-};
-
-// (the component goes through a scalar first: hipcc 7.2 compiles
-// __builtin_bit_cast of an ext_vector component as a cast of component 0)
-__device__ __forceinline__ float lo_f(u32x2_t v) {
-    const uint32_t t = v.x;
-    return __builtin_bit_cast(float, t);
-}
-__device__ __forceinline__ float hi_f(u32x2_t v) {
-    const uint32_t t = v.y;
-    return __builtin_bit_cast(float, t);
-}
-// lane l receives lane l-1's value (lane 0: 0) / lane l+1's value (lane 63: 0)
-__device__ __forceinline__ float from_prev_lane(uint32_t v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float from_next_lane(uint32_t v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xf, 0xf, false));
-}
-
-// Tile geometry requirements (checked by the launch): TX and TY even, so that
-// with the unclipped stream start S0 = x0 - (H-1) the active element of a
-// lane depends only on the launch's colour and the step and row parities.
-//
-// A narrow tile (nar) covers the last strip of z when it is at most TZN wide:
-// each half-wave holds a 64-voxel window of z, half 0 for the rows of y tile
-// 2*ty and half 1 for those of y tile 2*ty + 1 (TY rows further, an even
-// count, so the active element stays wave-uniform).  The DPP shifts mix the
-// halves only at lanes 31 and 32, which are halo lanes of both windows, and
-// the edge rows go through LDS per lane as before; only the addresses and
-// the stores (one per half) differ.
-template <int H, int KIND, int NW, int RPW>
-__device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
-                                            const uint8_t* __restrict__ Cg,
-                                            const float* __restrict__ hm, const Vol v, float w,
-                                            int colour0, int TX, int z0, int ty, int tx, bool nar) {
-    using Sh = RegShape<H, NW, RPW>;
-    constexpr int TY = Sh::TY, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
-    static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
-    static_assert(RPW % 2 == 0, "rows per wave must be even");
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* const sT = reinterpret_cast<float*>(smem);
-    float* const sE = reinterpret_cast<float*>(smem + Sh::TBL);
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // y tile of each half (for a wide tile half 1 owns nothing)
-    const int y0h[2] = {1 + (nar ? 2 * ty : ty) * TY, 1 + (nar ? 2 * ty + 1 : ty) * TY};
-    const int x0 = 1 + tx * TX;
-    const int x1 = min(x0 + TX, v.X + 1);
-    const int z1 = min(z0 + (nar ? Sh::TZN : Sh::TZ), v.Z + 1);
-    const int half = nar ? lane >> 5 : 0;                 // this lane's half
-    const int y0 = half ? y0h[1] : y0h[0];                  // this lane's y tile
-    const int ey0 = y0 - H;
-    const int ew = z0 - ML;                                 // odd
-    const int zl = ew + 2 * (nar ? (lane & 31) : lane);     // z of the lane's first voxel
-    const int lx1 = min(x1 + H, v.X + 2);
-
-    if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
-    } else if (KIND == TF_KIND_ANISOTROPIC) {
-        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    } else {
-        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    }
-    const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
-    const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
-
-    // ---- global memory (see sweep_tile for the descriptor conventions) ----
-    // loads: one 8-byte field load (the lane's pair) and one 2-byte code
-    // load per row and plane, straight into the register ring
-    constexpr int kOff = 0x7fffff00;
-    const int plane_f = v.sX * 4;
-    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
-                                (uint32_t)(v.sB * 4) + 2u * kSlackBytes);
-    const int row_c = sweep_code_row(v.Z);
-    const int plane_c = (v.Y + 2) * row_c;
-    const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
-    int vf[RPW], vc[RPW];
-    bool own_row[2][RPW];   // wave-uniform, per half
-    int ys[2][RPW];
-#pragma unroll
-    for (int j = 0; j < RPW; ++j) {
-        const int y = ey0 + wave * RPW + j;
-        const bool rok = y >= 0 && y <= v.Y + 1;
-        vf[j] = rok ? kSlackBytes + (y * v.sY + zl) * 4 : kOff;
-        vc[j] = rok ? y * row_c + zl + 3 : kOff;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int yk = y0h[k] - H + wave * RPW + j;
-            own_row[k][j] = (k == 0 || nar) && yk >= y0h[k] && yk < min(y0h[k] + TY, v.Y + 1);
-            ys[k][j] = own_row[k][j] ? yk : 0;
-        }
-    }
-    u32x2_t F[NR][RPW];
-    uint32_t K[NR][RPW];
-    // slots the prologue does not load hold planes before the stream start
-    // in the first steps: defined values, read by no active half-sweep
-#pragma unroll
-    for (int i = 0; i < NR; ++i)
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            F[i][j] = u32x2_t{0u, 0u};
-            K[i][j] = 0u;
-        }
-    auto issue_load = [&](int p, auto slot_c) {
-        constexpr int sl = decltype(slot_c)::value;
-        const bool pok = p >= 0 && p < lx1;
-        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, pok ? vf[j] : kOff, sf, 0);
-            K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
-        }
-    };
-    // store k writes the lanes of half k: one descriptor per row, plane and
-    // half, based at z = 0 of that row with record count 4*z1
-    const int zs0 = (zl >= z0 && half == 0) ? zl * 4 : kOff;
-    const int zs1 = (zl >= z0 && half == 1) ? zl * 4 : kOff;
-    auto store_plane = [&](int q, auto slot_c) {
-        constexpr int sl = decltype(slot_c)::value;
-        const bool qok = q >= x0 && q < x1;
-        const int qs = qok ? q : 0;
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const rsrc_t r = make_rsrc(D + qs * v.sX + ys[k][j] * v.sY,
-                                           (qok && own_row[k][j]) ? (uint32_t)(4 * z1) : 0u);
-                __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, k ? zs1 : zs0, 0, 0);
-            }
-        }
-    };
-
-    // ---- per-tile bounds of each half-sweep (planes only: rows and z are
-    // updated unconditionally; outside a half-sweep's region the result is
-    // never read by a later half-sweep of this pass nor stored, and ghosts
-    // (value 0 with code 0, whose table entry gives a zero quotient) keep
-    // their value) ----
-    int plo[H], pn[H];
-#pragma unroll
-    for (int h = 0; h < H; ++h) {
-        const int grow = H - 1 - h;
-        plo[h] = max(x0 - grow, 1);
-        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
-    }
-    // unclipped stream start: with TX and TY even, S0 + ey0 is odd, so the
-    // active element of row j at step u of an unrolled block,
-    // (s + y + colour0) & 1, is A ^ ((u + j) & 1) with A = colour0 ^ 1 for
-    // every tile of the launch; the parity of S0 is that of H
-    const int S0 = x0 - (H - 1);
-    const int S1 = (x1 - 1) + (H - 1);
-    const int A = (colour0 & 1) ^ 1;
-
-    // edge buffers: [h][parity][NW + 2][top, bottom][64 lanes]; wave w
-    // publishes in slot w + 1, so the missing neighbours of the first and
-    // last wave read a slot nobody writes (their rows are halo rows)
-    float* const eb = sE + (wave + 1) * 2 * kRowPairs + lane;
-    // parity of step s is that of S0 + u, S0 having the parity of H
-    auto ebuf = [&](int h, int par) { return eb + (h * 2 + par) * EB; };
-
-    auto body = [&](auto a_c) {
-        constexpr int AA = decltype(a_c)::value;
-        // active element (0: z = zl, 1: z = zl + 1) of row j at step u
-        auto act = [](int u, int j) { return (AA ^ (u + j)) & 1; };
-        auto par = [](int u) { return (H + u) & 1; };
-
-        auto publish = [&](int h, int u, auto slot_c) {
-            constexpr int sl = decltype(slot_c)::value;
-            float* e = ebuf(h, par(u));
-            const int a0 = act(u, 0), a1 = act(u, RPW - 1);
-            e[0] = a0 ? hi_f(F[sl][0]) : lo_f(F[sl][0]);
-            e[kRowPairs] = a1 ? hi_f(F[sl][RPW - 1]) : lo_f(F[sl][RPW - 1]);
-        };
-
-        auto stage = [&](auto h_c, auto u_c, int p) {
-            constexpr int h = decltype(h_c)::value, u = decltype(u_c)::value;
-            constexpr int sl = ((u - h + 1) % NR + NR) % NR;
-            constexpr int slm = (sl + NR - 1) % NR, slp = (sl + 1) % NR;
-            if ((unsigned)(p - plo[h]) >= (unsigned)pn[h]) return;
-            const float* e = ebuf(h, par(u + 1));
-            const float ym0 = e[-kRowPairs];        // bottom row of the wave above
-            const float ypL = e[2 * kRowPairs];     // top row of the wave below
-#pragma unroll
-            for (int j = 0; j < RPW; ++j) {
-                const int a = act(u, j);
-                const u32x2_t m = F[sl][j];
-                const float c = a ? hi_f(m) : lo_f(m);
-                const float xm = a ? hi_f(F[slm][j]) : lo_f(F[slm][j]);
-                const float xp = a ? hi_f(F[slp][j]) : lo_f(F[slp][j]);
-                const float ym = j > 0 ? (a ? hi_f(F[sl][j - 1]) : lo_f(F[sl][j - 1])) : ym0;
-                const float yp = j < RPW - 1 ? (a ? hi_f(F[sl][j + 1]) : lo_f(F[sl][j + 1])) : ypL;
-                const float zp = a ? from_next_lane(m.x) : hi_f(m);
-                const float zm = a ? lo_f(m) : from_prev_lane(m.y);
-                const uint32_t code = a ? (K[sl][j] >> 8) : (K[sl][j] & 0xffu);
-                float out;
-                if constexpr (KIND == TF_KIND_SOLVER) {
-                    const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
-                    float s = xp + xm;
-                    s = s + yp;
-                    s = s + ym;
-                    s = s + zp;
-                    s = s + zm;
-                    float qt = s * rdv.x;
-                    const float e2 = __builtin_fmaf(-qt, rdv.y, s);
-                    qt = __builtin_fmaf(e2, rdv.x, qt);
-                    if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
-                        qt = s / (code ? (float)code : __builtin_inff());
-                    s = qt - c;
-                    s = s * w;
-                    out = c + s;
-                } else if constexpr (KIND == TF_KIND_ELECTRODE) {
-                    float s = xp + xm;
-                    s = s + yp;
-                    s = s + ym;
-                    s = s + zp;
-                    s = s + zm;
-                    s = s / sT[code];
-                    s = s - c;
-                    s = s * w;
-                    out = c + s;
-                } else {
-                    float s = xp + xm;
-                    float t = yp + ym;
-                    t = ky * t;
-                    s = s + t;
-                    t = zp + zm;
-                    t = kz * t;
-                    s = s + t;
-                    s = s / sT[code];
-                    s = s - c;
-                    s = s * w;
-                    out = c + s;
-                }
-                if (a)
-                    F[sl][j].y = __builtin_bit_cast(uint32_t, out);
-                else
-                    F[sl][j].x = __builtin_bit_cast(uint32_t, out);
-            }
-        };
-
-        // planes S0-1 .. S0+1+DP in flight; plane S0's edges published as
-        // "step S0-1" for the first half-sweep
-        static_for<0, DP + 3>([&](auto i_c) {
-            constexpr int i = decltype(i_c)::value;
-            issue_load(S0 - 1 + i, ic<i>{});
-        });
-        publish(0, -1 + NR, ic<1>{});  // u = -1 has the parities of u = NR - 1
-        __syncthreads();
-
-        for (int s0 = S0; s0 <= S1; s0 += NR) {
-            static_for<0, NR>([&](auto u_c) {
-                constexpr int u = decltype(u_c)::value;
-                const int s = s0 + u;
-                static_for<0, H>([&](auto h_c) {
-                    constexpr int h = decltype(h_c)::value;
-                    stage(h_c, u_c, s - h);
-                    if constexpr (h + 1 < H) publish(h + 1, u, ic<((u - h + 1) % NR + NR) % NR>{});
-                });
-                publish(0, u, ic<(u + 2) % NR>{});
-                store_plane(s - (H - 1), ic<((u - H + 2) % NR + NR) % NR>{});
-                issue_load(s + 2 + DP, ic<((u - H + 1) % NR + NR) % NR>{});
-                __syncthreads();
-            });
-        }
-    };
-    if (A)
-        body(ic<1>{});
-    else
-        body(ic<0>{});
-}
-
-template <int H, int KIND, int NW, int RPW>
-__global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
-    // XCD-aware tile numbering (grid is a multiple of 8)
-    const int per = gridDim.x >> 3;
-    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (logical >= a.total) return;
-    int t = logical;
-    const bool nar = logical >= a.nWide;
-    int tz = 0, ty, tx, b;
-    if (!nar) {
-        tz = t % a.nTz; t /= a.nTz;
-        ty = t % a.nTy; t /= a.nTy;
-    } else {
-        t -= a.nWide;
-        ty = t % a.nTyN; t /= a.nTyN;
-    }
-    tx = t % a.nTx;
-    b = t / a.nTx;
-    const size_t off = (size_t)b * a.v.sB;
-    const size_t coff = KIND == TF_KIND_FIELD ? off
-                                              : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
-    const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
-    if constexpr (own_code_kind(KIND))
-        rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                      static_cast<const uint8_t*>(a.codes) + coff, hm, a.v, a.w,
-                                      a.colour0, a.TX,
-                                      nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx,
-                                      nar);
-    else
-        sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                     static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls,
-                                     a.v, a.w, a.colour0, a.TX, tz, ty, tx);
-}
-
-// Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows
-// and z voxels per tile, steps per unrolled block, LDS bytes.
-template <int H, int KIND, int NW, int RPW>
-struct SweepGeom {
-    static constexpr bool reg = own_code_kind(KIND);
-    static constexpr int TY = reg ? RegShape<H, NW, RPW>::TY : SweepShape<H, NW, RPW>::TY;
-    static constexpr int TZ = reg ? RegShape<H, NW, RPW>::TZ : SweepShape<H, NW, RPW>::TZ;
-    static constexpr int NP = reg ? RegShape<H, NW, RPW>::NR : SweepShape<H, NW, RPW>::NP;
-};
-
-template <int H, int NW, int RPW>
-static size_t sweep_lds_bytes(int kind, int ncls) {
-    if (own_code_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes();
-    using Sh = SweepShape<H, NW, RPW>;
-    const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
-    size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
-    bytes += kind == TF_KIND_MULTIPHASE    ? (size_t)ncls * ncls * sizeof(float)
-             : kind == TF_KIND_ANISOTROPIC ? TF_ANISO_TABLE * sizeof(float)
-             : kind == TF_KIND_ELECTRODE   ? TF_ELECTRODE_TABLE * sizeof(float)
-                                           : 18 * sizeof(float);
-    return bytes;
-}
-
-// Workgroups of one sweep kernel resident on the current device at once
-// (CUs x workgroups per CU); 0 if the runtime cannot say.  `cache` holds the
-// answer + 1 per device (0: not asked yet).
-constexpr int kMaxDevices = 64;
-static int resident_workgroups(const void* fn, int threads, size_t lds, int* cache) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
-        (void)hipGetLastError();
-        return 0;
-    }
-    if (cache[dev] > 0) return cache[dev] - 1;
-    int cus = 0, per_cu = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds) != hipSuccess) {
-        (void)hipGetLastError();
-        cus = per_cu = 0;
-    }
-    cache[dev] = std::max(0, cus) * std::max(0, per_cu) + 1;
-    return cache[dev] - 1;
-}
-
-// x-tile length that minimises the modelled duration of one launch: the
-// tiles are spread evenly over the 8 XCDs (k_sweep's numbering) and run in
-// rounds of the resident workgroups; a tile costs its streamed planes,
-// TX + 2(H-1) rounded up to the unrolled step count NP, plus a fixed start.
-// A grid with fewer tiles than resident slots is one round, so for small
-// volumes the shortest tile that still fits one round wins.
-static int balanced_tx(int X, long long yz_tiles, int H, int NP, int slots, bool even) {
-    auto fit = [even](int tx) { return even ? (tx + 1) & ~1 : tx; };
-    if (slots <= 0) return fit(std::min(X, 64));
-    const long long per_xcd = std::max(1, slots / 8);
-    long long best_cost = -1;
-    int best = fit(X);
-    for (int n = 1; n <= X; ++n) {
-        const int tx = fit((X + n - 1) / n);
-        const int nx = (X + tx - 1) / tx;
-        if (nx != n) continue;  // same tiling as a smaller n
-        const long long tiles = yz_tiles * nx;
-        const long long rounds = ((tiles + 7) / 8 + per_xcd - 1) / per_xcd;
-        const long long steps = (tx + 2 * (H - 1) + NP - 1) / NP * NP + 4;
-        const long long cost = rounds * steps;
-        if (best_cost < 0 || cost < best_cost) {
-            best_cost = cost;
-            best = tx;
-        }
-        if (tx <= 8) break;
-    }
-    return best;
-}
-
-template <int H, int KIND, int NW, int RPW>
-static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
-    using Sh = SweepGeom<H, KIND, NW, RPW>;
-    if constexpr (Sh::TY < (Sh::reg ? 2 : 1)) {
-        return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
-    } else {
-        const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
-        if (lds > 160 * 1024)
-            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
-        static bool attr_set = false;
-        if (!attr_set && lds > 64 * 1024) {
-            // opt in to more than 64 KiB of dynamic LDS; if the runtime
-            // refuses, the launch below reports the error
-            if (hipFuncSetAttribute((const void*)k_sweep<H, KIND, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
-                (void)hipGetLastError();
-            attr_set = true;
-        }
-        a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
-        a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
-        a.nTyN = 0;
-        if constexpr (Sh::reg) {
-            // wide tiles while a full one fits, then a narrow strip if the
-            // rest fits one (RegShape::TZN), else one more wide tile
-            const int tzn = RegShape<H, NW, RPW>::TZN;
-            const int wide = a.v.Z / Sh::TZ, rest = a.v.Z - wide * Sh::TZ;
-            a.nTz = wide + (rest > tzn ? 1 : 0);
-            if (rest > 0 && rest <= tzn) a.nTyN = (a.nTy + 1) / 2;
-            a.z0N = 1 + wide * Sh::TZ;
-        }
-        const long long yz = (long long)a.nTz * a.nTy + a.nTyN;
-        if (a.TX <= 0) {
-            static int slots_cache[kMaxDevices] = {};
-            const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
-                                                  lds, slots_cache);
-            a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, Sh::reg);
-        }
-        // the register-blocked kernel needs an even x tile (rsweep_tile)
-        if (Sh::reg) a.TX = (a.TX + 1) & ~1;
-        a.nTx = (a.v.X + a.TX - 1) / a.TX;
-        const long long wide_tiles = (long long)a.nTz * a.nTy * a.nTx * B;
-        const long long total = yz * a.nTx * B;
-        if (total * 8 >= (1LL << 31)) return fail(-1, "too many tiles (%lld)", total);
-        a.nWide = (int)wide_tiles;
-        a.total = (int)total;
-        if (a.resolve_only) return 0;
-        const int grid = (int)(((total + 7) / 8) * 8);
-        hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
-        TF_LAUNCH_CHECK("k_sweep");
-        return 0;
-    }
-}
-
-// workgroup shapes (waves x rows per wave) that are compiled
-// compiled workgroup shapes: all five for TF_KIND_SOLVER, 8x2 / 4x4 / 16x2
-// for the other kinds
-template <int H, int KIND>
-static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
-    if (nw == 8 && rpw == 2) return launch_sweep<H, KIND, 8, 2>(a, B, st);
-    if (nw == 16 && rpw == 2) return launch_sweep<H, KIND, 16, 2>(a, B, st);
-    if constexpr (H <= 4) {
-        if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
-    }
-    if constexpr (KIND == TF_KIND_SOLVER) {
-        if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
-        if constexpr (H <= 4) {
-            if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
-        }
-    }
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for kind %d and n_half %d",
-                nw, rpw, KIND, H);
-}
-
-template <int KIND>
-static int dispatch_sweep(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st) {
-    switch (H) {
-        case 1: return dispatch_shape<1, KIND>(a, B, nw, rpw, st);
-        case 2: return dispatch_shape<2, KIND>(a, B, nw, rpw, st);
-        case 3: return dispatch_shape<3, KIND>(a, B, nw, rpw, st);
-        case 4: return dispatch_shape<4, KIND>(a, B, nw, rpw, st);
-        case 5: return dispatch_shape<5, KIND>(a, B, nw, rpw, st);
-        case 6: return dispatch_shape<6, KIND>(a, B, nw, rpw, st);
-        default: return fail(-1, "n_half=%d out of range 1..%d", H, TF_MAX_FUSED_HALF_SWEEPS);
-    }
-}
-
 // owned voxels of z per tile of the register-blocked kernel (RegShape::TZ)
 static int reg_tile_tz(int H) { return ((kRowLen - (H + (H & 1)) - H) / 2) * 2; }
 
@@ -1506,14 +443,14 @@ static int sweep_prepare(const float* hm_lut, int n_classes, int kind, int B, in
 static int sweep_dispatch(int kind, int n_half, SweepArgs& a, int B, const tf_tile& t,
                           hipStream_t st) {
     if (kind == TF_KIND_SOLVER)
-        return dispatch_sweep<TF_KIND_SOLVER>(n_half, a, B, t.waves, t.rows_per_wave, st);
+        return sweep_dispatch_solver(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_MULTIPHASE)
-        return dispatch_sweep<TF_KIND_MULTIPHASE>(n_half, a, B, t.waves, t.rows_per_wave, st);
+        return sweep_dispatch_multiphase(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_ANISOTROPIC)
-        return dispatch_sweep<TF_KIND_ANISOTROPIC>(n_half, a, B, t.waves, t.rows_per_wave, st);
+        return sweep_dispatch_anisotropic(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_ELECTRODE)
-        return dispatch_sweep<TF_KIND_ELECTRODE>(n_half, a, B, t.waves, t.rows_per_wave, st);
-    return dispatch_sweep<TF_KIND_FIELD>(n_half, a, B, t.waves, t.rows_per_wave, st);
+        return sweep_dispatch_electrode(n_half, a, B, t.waves, t.rows_per_wave, st);
+    return sweep_dispatch_field(n_half, a, B, t.waves, t.rows_per_wave, st);
 }
 
 extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const float* hm_lut,
@@ -1892,88 +829,6 @@ extern "C" int tf_grad_classes(const float* field, const uint8_t* codes, const f
     return 0;
 }
 
-// ---------------------------------------------------------------------------
-// uneven batch: many two-phase volumes, each with its own convergence
-// ---------------------------------------------------------------------------
-// Volumes live back to back in two packed field buffers and one packed code
-// buffer.  A batch pass covers the tiles of every volume in one launch;
-// tiles of volumes that have stopped return at once.  Every active volume has
-// done the same number of iterations (they all start at 0 and advance
-// together), so one colour and one source buffer hold for the whole launch.
-// The per-volume convergence decision runs on the device (k_batch_check), so
-// a sequence of passes and checks needs no host round trip and can be
-// captured in a graph.
-struct BatchArgs {
-    float* fa;
-    float* fb;
-    const uint8_t* codes;
-    const tf_volume* vols;
-    const int* tile_vol;
-    const tf_volume_state* state;
-    int n_tiles;
-    int src_b;
-    int colour0;
-    int TX;
-};
-
-template <int H, int NW, int RPW>
-__global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
-    using Sh = RegShape<H, NW, RPW>;
-    const int per = gridDim.x >> 3;
-    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (logical >= a.n_tiles) return;
-    const int vi = a.tile_vol[logical];
-    if (!a.state[vi].active) return;
-    const tf_volume vol = a.vols[vi];
-    const Vol v = make_vol(vol.X, vol.Y, vol.Z);
-    const int nTz = (vol.Z + Sh::TZ - 1) / Sh::TZ;
-    const int nTy = (vol.Y + Sh::TY - 1) / Sh::TY;
-    int t = logical - vol.tile_first;
-    const int tz = t % nTz; t /= nTz;
-    const int ty = t % nTy;
-    const int tx = t / nTy;
-    const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
-    float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
-    rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
-                                            vol.omega, a.colour0, a.TX, 1 + tz * Sh::TZ, ty, tx,
-                                            false);
-}
-
-template <int H, int NW, int RPW>
-static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
-    using Sh = RegShape<H, NW, RPW>;
-    if constexpr (Sh::TY < 2) {
-        return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
-    } else {
-        const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
-        if (lds > 160 * 1024)
-            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
-        const int grid = ((a.n_tiles + 7) / 8) * 8;
-        static bool attr_set = false;
-        if (!attr_set && lds > 64 * 1024) {
-            if (hipFuncSetAttribute((const void*)k_sweep_batch<H, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
-                (void)hipGetLastError();
-            attr_set = true;
-        }
-        hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
-        TF_LAUNCH_CHECK("k_sweep_batch");
-        return 0;
-    }
-}
-
-template <int H>
-static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
-    if (nw == 8 && rpw == 2) return launch_sweep_batch<H, 8, 2>(a, st);
-    if (nw == 12 && rpw == 2) return launch_sweep_batch<H, 12, 2>(a, st);
-    if (nw == 16 && rpw == 2) return launch_sweep_batch<H, 16, 2>(a, st);
-    if constexpr (H <= 4) {
-        if (nw == 4 && rpw == 4) return launch_sweep_batch<H, 4, 4>(a, st);
-        if (nw == 8 && rpw == 4) return launch_sweep_batch<H, 8, 4>(a, st);
-    }
-    return fail(-1, "workgroup shape %d waves x %d rows is not compiled for n_half %d", nw, rpw, H);
-}
-
 static int resolve_tile(int n_half, const tf_tile* tile, tf_tile* t) {
     // batches: volumes of different sizes share one shape; 16-row tiles
     t->waves = 8;
@@ -2025,14 +880,7 @@ extern "C" int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* cod
     a.state = state; a.n_tiles = n_tiles; a.src_b = src_buffer; a.colour0 = first_colour;
     a.TX = t.tx;
     hipStream_t st = (hipStream_t)stream;
-    switch (n_half) {
-        case 1: return dispatch_batch_shape<1>(a, t.waves, t.rows_per_wave, st);
-        case 2: return dispatch_batch_shape<2>(a, t.waves, t.rows_per_wave, st);
-        case 3: return dispatch_batch_shape<3>(a, t.waves, t.rows_per_wave, st);
-        case 4: return dispatch_batch_shape<4>(a, t.waves, t.rows_per_wave, st);
-        case 5: return dispatch_batch_shape<5>(a, t.waves, t.rows_per_wave, st);
-        default: return dispatch_batch_shape<6>(a, t.waves, t.rows_per_wave, st);
-    }
+    return sweep_dispatch_batch(n_half, a, t.waves, t.rows_per_wave, st);
 }
 
 __global__ __launch_bounds__(kSumThreads) void k_batch_layer_sums(
