@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 4
+#define TF_ABI_VERSION 5
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -181,8 +181,9 @@ int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int B, int X
  * equals n_half reference iterations value for value.
  *   src, dst field buffers with TF_FIELD_SLACK_BYTES of slack
  *   codes    TF_KIND_SOLVER / TF_KIND_MULTIPHASE: the sweep-code layout
- *            (tf_pack_sweep_codes); TF_KIND_FIELD: the padded per-voxel
- *            diffusivity with TF_FIELD_SLACK_BYTES of slack
+ *            (tf_pack_sweep_codes); TF_KIND_FIELD: the face conductances
+ *            of tf_field_faces (three sections, TF_FIELD_SLACK_BYTES of
+ *            slack before the first and after the last)
  *   hm_lut   device float [n_classes*n_classes] face conductances
  *            (TF_KIND_MULTIPHASE), ignored for TF_KIND_SOLVER
  *   omega    the over-relaxation factor rounded to float32
@@ -236,6 +237,17 @@ int tf_layer_sums(const float* field, const void* codes,
 int tf_setup_field(const float* k, const float* xinit, float* kpad,
                    float* field_a, float* field_b, uint32_t* layer_count,
                    int B, int X, int Y, int Z, void* stream);
+
+/* Face conductances of the per-voxel field for tf_sweep (TF_KIND_FIELD):
+ * faces = 3 consecutive sections of B*(X+2)*(Y+2)*(Z+2) floats (gx, gy, gz),
+ * each face stored at the padded voxel on its minus side:
+ * gx[o] = hm(kpad[o], kpad[o + x]) and likewise for y and z, 0 where that
+ * neighbour is outside the padded volume; hm(a, b) = ((2a) b)/(a + b), 0 for
+ * a + b = 0.  Computing each face once replaces the six harmonic means of
+ * every update (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:
+ * 577-583, per voxel pair) with the same float32 values.
+ *   kpad   the padded diffusivity of tf_setup_field                         */
+int tf_field_faces(const float* kpad, float* faces, int B, int X, int Y, int Z, void* stream);
 
 /* Gradient of D_rel with respect to every voxel diffusivity, from a
  * converged field (no reference counterpart: the reference solves under

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -56,6 +56,7 @@ SIGNATURES = {
                                _vp, _vp, _vp]),
     "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "tf_setup_field": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_field_faces": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_grad_field": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "tf_grad_classes": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp]),
     "tf_sweep_launch_config": (_c_int, [_c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,

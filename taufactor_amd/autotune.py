@@ -29,7 +29,7 @@ TIMED_LAUNCHES = 3
 # stable against timing noise.  x tile 0 is the library's occupancy-balanced
 # length for the shape and volume.
 SOLVER_SPACE = ((5, 4, 6, 3, 2), ((16, 2), (12, 2), (8, 2)), (0, 128, 64))
-FIELD_SPACE = ((4, 3, 2), ((8, 2), (4, 4)), (0, 128, 64))
+FIELD_SPACE = ((4, 3, 2), ((8, 2), (12, 2), (16, 2)), (0, 128, 64))
 BATCH_SPACE = ((4, 2), ((8, 2), (4, 4), (12, 2), (16, 2)), (64, 32))
 TIE = 0.02
 

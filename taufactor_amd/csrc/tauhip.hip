@@ -467,6 +467,7 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     a.w = omega;
     a.colour0 = first_colour;
     a.TX = t.tx;
+    a.fsec = (long long)B * a.v.sB;
     return sweep_dispatch(kind, n_half, a, B, t, (hipStream_t)stream);
 }
 
@@ -681,6 +682,37 @@ extern "C" int tf_setup_field(const float* k, const float* xinit, float* kpad, f
     TF_LAUNCH_CHECK("k_setup_field");
     hipLaunchKernelGGL(k_layer_count_field, dim3(X, B), dim3(256), 0, st, k, layer_count, v);
     TF_LAUNCH_CHECK("k_layer_count_field");
+    return 0;
+}
+
+// Face conductances of the per-voxel field for the sweep: for every padded
+// voxel o, gx[o] = hm(k[o], k[o + x]), gy[o] = hm(k[o], k[o + y]),
+// gz[o] = hm(k[o], k[o + z]) (0 where the neighbour is outside the padded
+// volume).  hm(a, b) = ((2a) b)/(a + b) rounds the same either way round
+// (doubling is exact, the product is rounded once), so the face of voxels u
+// and v is what the update of either computes (MultiPhaseSolver's harmonic
+// mean, taufactor/taufactor.py:577-583).
+__global__ void k_field_faces(const float* __restrict__ kpad, float* __restrict__ faces, Vol v, int B,
+                              long long fsec) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)B * v.sB) return;
+    const int r = (int)(gid % v.sB);
+    const int xp = r / v.sX, yp = (r - xp * v.sX) / v.sY, zp = r - xp * v.sX - yp * v.sY;
+    const float kc = kpad[gid];
+    faces[gid] = xp <= v.X ? hmean(kc, kpad[gid + v.sX]) : 0.0f;
+    faces[fsec + gid] = yp <= v.Y ? hmean(kc, kpad[gid + v.sY]) : 0.0f;
+    faces[2 * fsec + gid] = zp <= v.Z ? hmean(kc, kpad[gid + 1]) : 0.0f;
+}
+
+extern "C" int tf_field_faces(const float* kpad, float* faces, int B, int X, int Y, int Z,
+                              void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!kpad || !faces) return fail(-1, "null pointer argument");
+    const Vol v = make_vol(X, Y, Z);
+    const long long total = (long long)B * v.sB;
+    hipLaunchKernelGGL(k_field_faces, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, kpad, faces, v, B, total);
+    TF_LAUNCH_CHECK("k_field_faces");
     return 0;
 }
 

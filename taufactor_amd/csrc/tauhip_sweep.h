@@ -221,6 +221,7 @@ struct SweepArgs {
     // register-blocked kinds: tiles [0, nWide) are wide, the rest narrow
     // (the last strip of z, from z0N, nTyN tiles of two y tiles each)
     int nWide, nTyN, z0N;
+    long long fsec;    // field kind: floats between the gx, gy, gz face sections
     int resolve_only;  // set TX and the tile counts, launch nothing
 };
 
@@ -608,6 +609,12 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
 __host__ __device__ constexpr bool own_code_kind(int kind) {
     return kind == TF_KIND_SOLVER || kind == TF_KIND_ANISOTROPIC || kind == TF_KIND_ELECTRODE;
 }
+// kinds swept by the register-blocked kernel: the own-code kinds and the
+// per-voxel field kind, whose face conductances are precomputed
+// (tf_field_faces)
+__host__ __device__ constexpr bool reg_kind(int kind) {
+    return own_code_kind(kind) || kind == TF_KIND_FIELD;
+}
 
 template <int H, int NW, int RPW>
 struct RegShape {
@@ -622,7 +629,12 @@ struct RegShape {
     static constexpr int D = NR - H - 2;
     static constexpr int TBL = 640;                           // LDS bytes for the table
     static constexpr int EB = (NW + 2) * 2 * kRowPairs;       // floats per edge buffer
-    static constexpr size_t lds_bytes() { return TBL + (size_t)H * 2 * EB * sizeof(float); }
+    // field kind: the y faces of each wave's last row, per ring plane
+    static constexpr int GEB = (NW + 2) * 2 * kRowPairs;     // floats per ring plane
+    static constexpr size_t lds_bytes(int kind) {
+        return TBL + (size_t)H * 2 * EB * sizeof(float) +
+               (kind == TF_KIND_FIELD ? (size_t)NR * GEB * sizeof(float) : 0);
+    }
 };
 
 // (the component goes through a scalar first: hipcc 7.2 compiles
@@ -658,14 +670,21 @@ template <int H, int KIND, int NW, int RPW>
 __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
-                                            int colour0, int TX, int z0, int ty, int tx, bool nar) {
+                                            int colour0, int TX, int z0, int ty, int tx, bool nar,
+                                            long long fsec = 0) {
     using Sh = RegShape<H, NW, RPW>;
+    // field kind: Cg holds three sections of face conductances in the padded
+    // field layout, fsec floats apart: gx, gy, gz, each stored at the voxel
+    // on the face's minus side (tf_field_faces)
+    constexpr bool kFaces = KIND == TF_KIND_FIELD;
+    constexpr int NG = kFaces ? Sh::NR : 1;
     constexpr int TY = Sh::TY, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
     static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
     static_assert(RPW % 2 == 0, "rows per wave must be even");
     extern __shared__ __align__(16) unsigned char smem[];
     float* const sT = reinterpret_cast<float*>(smem);
     float* const sE = reinterpret_cast<float*>(smem + Sh::TBL);
+    float* const sGE = sE + (size_t)H * 2 * EB;  // field kind: y faces of edge rows
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -685,7 +704,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
     } else if (KIND == TF_KIND_ANISOTROPIC) {
         if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    } else {
+    } else if (KIND == TF_KIND_ELECTRODE) {
         if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
     }
     const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
@@ -701,6 +720,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int row_c = sweep_code_row(v.Z);
     const int plane_c = (v.Y + 2) * row_c;
     const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
+    rsrc_t rG[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        rG[k] = make_rsrc(reinterpret_cast<const char*>(reinterpret_cast<const float*>(Cg) + k * fsec) -
+                              kSlackBytes,
+                          (uint32_t)(v.sB * 4) + 2u * kSlackBytes);
     int vf[RPW], vc[RPW];
     bool own_row[2][RPW];   // wave-uniform, per half
     int ys[2][RPW];
@@ -719,6 +744,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     }
     u32x2_t F[NR][RPW];
     uint32_t K[NR][RPW];
+    u32x2_t G[NG][RPW][3];  // field kind: gx, gy, gz pairs
     // slots the prologue does not load hold planes before the stream start
     // in the first steps: defined values, read by no active half-sweep
 #pragma unroll
@@ -728,6 +754,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             F[i][j] = u32x2_t{0u, 0u};
             K[i][j] = 0u;
         }
+#pragma unroll
+    for (int i = 0; i < NG; ++i)
+#pragma unroll
+        for (int j = 0; j < RPW; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) G[i][j][k] = u32x2_t{0u, 0u};
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         const bool pok = p >= 0 && p < lx1;
@@ -735,7 +767,13 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, pok ? vf[j] : kOff, sf, 0);
-            K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
+            if constexpr (kFaces) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    G[sl][j][k] = __builtin_amdgcn_raw_buffer_load_b64(rG[k], pok ? vf[j] : kOff, sf, 0);
+            } else {
+                K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
+            }
         }
     };
     // store k writes the lanes of half k: one descriptor per row, plane and
@@ -797,6 +835,16 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             e[0] = a0 ? hi_f(F[sl][0]) : lo_f(F[sl][0]);
             e[kRowPairs] = a1 ? hi_f(F[sl][RPW - 1]) : lo_f(F[sl][RPW - 1]);
         };
+        // field kind: the y faces of the last row of a freshly loaded plane,
+        // read by the first row of the wave below in every half-sweep of
+        // that plane (LDS ring slot = register ring slot)
+        auto publish_faces = [&](auto slot_c) {
+            if constexpr (kFaces) {
+                constexpr int sl = decltype(slot_c)::value;
+                reinterpret_cast<u32x2_t*>(sGE + sl * Sh::GEB)[(wave + 1) * kRowPairs + lane] =
+                    G[sl][RPW - 1][1];
+            }
+        };
 
         auto stage = [&](auto h_c, auto u_c, int p) {
             constexpr int h = decltype(h_c)::value, u = decltype(u_c)::value;
@@ -806,6 +854,9 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             const float* e = ebuf(h, par(u + 1));
             const float ym0 = e[-kRowPairs];        // bottom row of the wave above
             const float ypL = e[2 * kRowPairs];     // top row of the wave below
+            u32x2_t gy0{0u, 0u};                    // y faces of the wave above's last row
+            if constexpr (kFaces)
+                gy0 = reinterpret_cast<const u32x2_t*>(sGE + sl * Sh::GEB)[wave * kRowPairs + lane];
 #pragma unroll
             for (int j = 0; j < RPW; ++j) {
                 const int a = act(u, j);
@@ -844,6 +895,19 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     s = s - c;
                     s = s * w;
                     out = c + s;
+                } else if constexpr (KIND == TF_KIND_FIELD) {
+                    // faces of the voxel: its own gx, gy, gz and those of its
+                    // x-1 (previous plane), y-1 (row above) and z-1 (other
+                    // element or previous lane) neighbours
+                    auto sel = [a](u32x2_t q) { return a ? hi_f(q) : lo_f(q); };
+                    const u32x2_t gz = G[sl][j][2];
+                    const float gxp = sel(G[sl][j][0]), gxm = sel(G[slm][j][0]);
+                    const float gyp = sel(G[sl][j][1]);
+                    const float gym = sel(j > 0 ? G[sl][j - 1][1] : gy0);
+                    const float gzp = sel(gz);
+                    const float gzm = a ? lo_f(gz) : from_prev_lane(gz.y);
+                    const Nbr n{c, xp, xm, yp, ym, zp, zm};
+                    out = update_multiphase(n, gxp, gxm, gyp, gym, gzp, gzm, p == 1, p == v.X, w);
                 } else {
                     float s = xp + xm;
                     float t = yp + ym;
@@ -871,6 +935,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             issue_load(S0 - 1 + i, ic<i>{});
         });
         publish(0, -1 + NR, ic<1>{});  // u = -1 has the parities of u = NR - 1
+        publish_faces(ic<1>{});
         __syncthreads();
 
         for (int s0 = S0; s0 <= S1; s0 += NR) {
@@ -883,6 +948,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     if constexpr (h + 1 < H) publish(h + 1, u, ic<((u - h + 1) % NR + NR) % NR>{});
                 });
                 publish(0, u, ic<(u + 2) % NR>{});
+                publish_faces(ic<(u + 2) % NR>{});
                 store_plane(s - (H - 1), ic<((u - H + 2) % NR + NR) % NR>{});
                 issue_load(s + 2 + DP, ic<((u - H + 1) % NR + NR) % NR>{});
                 __syncthreads();
@@ -917,12 +983,13 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const size_t coff = KIND == TF_KIND_FIELD ? off
                                               : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
-    if constexpr (own_code_kind(KIND))
+    if constexpr (reg_kind(KIND))
         rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                      static_cast<const uint8_t*>(a.codes) + coff, hm, a.v, a.w,
-                                      a.colour0, a.TX,
+                                      reinterpret_cast<const uint8_t*>(
+                                          static_cast<const code_t<KIND>*>(a.codes) + coff),
+                                      hm, a.v, a.w, a.colour0, a.TX,
                                       nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx,
-                                      nar);
+                                      nar, a.fsec);
     else
         sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
                                      static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls,
@@ -933,7 +1000,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
 // and z voxels per tile, steps per unrolled block, LDS bytes.
 template <int H, int KIND, int NW, int RPW>
 struct SweepGeom {
-    static constexpr bool reg = own_code_kind(KIND);
+    static constexpr bool reg = reg_kind(KIND);
     static constexpr int TY = reg ? RegShape<H, NW, RPW>::TY : SweepShape<H, NW, RPW>::TY;
     static constexpr int TZ = reg ? RegShape<H, NW, RPW>::TZ : SweepShape<H, NW, RPW>::TZ;
     static constexpr int NP = reg ? RegShape<H, NW, RPW>::NR : SweepShape<H, NW, RPW>::NP;
@@ -941,7 +1008,7 @@ struct SweepGeom {
 
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
-    if (own_code_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes();
+    if (reg_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes(kind);
     using Sh = SweepShape<H, NW, RPW>;
     const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
     size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
@@ -1065,11 +1132,11 @@ static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) 
     if constexpr (H <= 4) {
         if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
     }
-    if constexpr (KIND == TF_KIND_SOLVER) {
+    if constexpr (KIND == TF_KIND_SOLVER || KIND == TF_KIND_FIELD) {
         if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
-        if constexpr (H <= 4) {
-            if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
-        }
+    }
+    if constexpr (KIND == TF_KIND_SOLVER && H <= 4) {
+        if (nw == 8 && rpw == 4) return launch_sweep<H, KIND, 8, 4>(a, B, st);
     }
     return fail(-1, "workgroup shape %d waves x %d rows is not compiled for kind %d and n_half %d",
                 nw, rpw, KIND, H);

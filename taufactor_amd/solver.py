@@ -803,6 +803,8 @@ class FieldSolver(ThroughTransportSolver):
                 np.ascontiguousarray(self.cpu_img, dtype=np.float32)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = _lib.slack_buffer(shape, torch.float32, self.device)
+        # the sweep reads precomputed face conductances (gx, gy, gz sections)
+        self._sweep_codes = _lib.slack_buffer((3,) + shape, torch.float32, self.device)
         self._fields = [_lib.slack_buffer(shape, torch.float32, self.device) for _ in range(2)]
         self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
         self._xinit_dev = self._xinit()
@@ -814,6 +816,9 @@ class FieldSolver(ThroughTransportSolver):
                                       _lib.ptr(self._fields[1]), _lib.ptr(self._count),
                                       B, X, Y, Z, self._stream())
         _lib.check(st, "tf_setup_field")
+        st = self._lib.tf_field_faces(_lib.ptr(self._codes), _lib.ptr(self._sweep_codes), B, X, Y, Z,
+                                      self._stream())
+        _lib.check(st, "tf_field_faces")
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     def grad_D_eff(self) -> torch.Tensor:

@@ -39,6 +39,20 @@ def test_field_iterates_bit_identical(shape, H):
     assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
 
 
+@pytest.mark.parametrize("tile", [(8, 2, 6), (12, 2, 0), (16, 2, 4)])
+@pytest.mark.parametrize("H", [2, 3, 4])
+def test_field_iterates_with_tiles_and_batch(tile, H):
+    """Two volumes at once, every compiled field shape: the register-blocked
+    sweep reads the precomputed face conductances (tf_field_faces)."""
+    k = np.stack([_k((11, 30, 70), 40 + H, zero_frac=0.25), _k((11, 30, 70), 41 + H)])
+    s = tau.FieldSolver(k)
+    s.half_sweeps_per_pass = H
+    s.tile = tile
+    s.solve(iter_limit=2 * H + 3, verbose=False)
+    o = OracleFieldSolver(k).run(2 * H + 3)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
 def test_field_full_solve_matches_oracle():
     k = _k((28, 20, 24), 3, zero_frac=0.2)
     s = tau.FieldSolver(k)

@@ -197,3 +197,21 @@ def test_integration_stub_runs():
     flux, conc = st.layer_means()
     vf = o.vertical_flux().to(torch.float64).mean((2, 3)).numpy()
     np.testing.assert_allclose(flux, vf, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("H,Z", [(4, 120), (4, 176), (4, 177), (4, 240), (5, 168), (5, 169), (3, 7)])
+@pytest.mark.parametrize("Y", [40, 60])
+def test_solver_z_tiling_boundaries(H, Z, Y):
+    """The register-blocked sweep covers z with wide tiles of TZ voxels and, when
+    the rest fits, one narrow strip of at most TZN (TZ = 120, TZN = 56 at
+    n_half = 4; 116 and 52 at 5): Z = TZ (wide only), TZ + TZN (narrow strip of
+    TZN), TZ + TZN + 1 (two wide), 2 TZ, and small Z (narrow only).  With 16x2
+    workgroups Y = 60 gives an odd count of y tiles, so the last narrow tile has
+    an empty half."""
+    img = _binary((9, Y, Z), seed=Z + Y + H)
+    s = tau.Solver(img)
+    s.half_sweeps_per_pass = H
+    s.tile = (16, 2, 4)
+    s.solve(iter_limit=2 * H + 1, verbose=False)
+    o = OracleSolver(img).run(2 * H + 1)
+    assert np.array_equal(_field(s), o.field.numpy())
