@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 5
+#define TF_ABI_VERSION 6
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -166,9 +166,12 @@ int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream);
 long long tf_sweep_codes_bytes(int X, int Y, int Z);
 
 /* Repack codes (layout above, B volumes) into the sweep-code layout
- * (B consecutive volumes of tf_sweep_codes_bytes each; 4-byte aligned).  */
-int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int B, int X, int Y, int Z,
-                        void* stream);
+ * (B consecutive volumes of tf_sweep_codes_bytes each; 4-byte aligned) for
+ * tf_sweep / tf_batch_sweep of the given kind: TF_KIND_SOLVER stores each
+ * count times 8 (the byte offset of its (1/d, d) table entry, at most 64),
+ * the other kinds the code itself.  */
+int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int kind, int B, int X, int Y,
+                        int Z, void* stream);
 
 /* n_half over-relaxed red-black half-sweeps (reference iterations), the
  * first one on colour first_colour, fused in one HBM pass: reads src,

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -47,7 +47,7 @@ SIGNATURES = {
                                     _c_int, _vp]),
     "tf_wrap_ghosts": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep_codes_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int]),
-    "tf_pack_sweep_codes": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_pack_sweep_codes": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                           _c_float, _c_int, _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_sweep_half_inplace": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,

@@ -132,7 +132,7 @@ class BatchSolver:
             _lib.check(st, "tf_setup_solver")
             st = self._lib.tf_pack_sweep_codes(
                 _lib.ptr(self._codes) + o, _lib.ptr(self._sweep_codes) + int(self._vols_host[v]["code_offset"]),
-                1, X, Y, Z, self._stream())
+                _lib.KIND_SOLVER, 1, X, Y, Z, self._stream())
             _lib.check(st, "tf_pack_sweep_codes")
 
     def reset(self):

@@ -301,7 +301,7 @@ extern "C" int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conduc
 
 // one thread per 4-byte word of the sweep-code layout (rows are whole words)
 __global__ void k_pack_sweep_codes(const uint8_t* __restrict__ codes, uint32_t* __restrict__ out,
-                                   Vol v, int row_c, long long n_words) {
+                                   Vol v, int row_c, long long n_words, int shift) {
     const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= n_words) return;
     const long long byte0 = gid * 4;
@@ -313,7 +313,7 @@ __global__ void k_pack_sweep_codes(const uint8_t* __restrict__ codes, uint32_t* 
     for (int i = 0; i < 4; ++i) {
         const int z = k0 + i - 3;
         const uint32_t c = (z >= 0 && z <= v.Z + 1) ? src[z] : 0u;
-        w |= c << (8 * i);
+        w |= (c << shift) << (8 * i);
     }
     out[gid] = w;
 }
@@ -323,9 +323,10 @@ extern "C" long long tf_sweep_codes_bytes(int X, int Y, int Z) {
     return sweep_code_volume(X, Y, Z);
 }
 
-extern "C" int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int B, int X, int Y,
-                                   int Z, void* stream) {
+extern "C" int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int kind, int B, int X,
+                                   int Y, int Z, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (kind < TF_KIND_SOLVER || kind > TF_KIND_ELECTRODE) return fail(-1, "unknown kind %d", kind);
     if (!codes || !sweep_codes) return fail(-1, "null pointer argument");
     if (reinterpret_cast<uintptr_t>(sweep_codes) & 3) return fail(-1, "sweep_codes must be 4-byte aligned");
     const long long n_words = (long long)B * sweep_code_volume(X, Y, Z) / 4;
@@ -333,7 +334,7 @@ extern "C" int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, i
     if (blocks >= (1LL << 31)) return fail(-1, "volume too large");
     hipLaunchKernelGGL(k_pack_sweep_codes, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                        codes, reinterpret_cast<uint32_t*>(sweep_codes), make_vol(X, Y, Z),
-                       sweep_code_row(Z), n_words);
+                       sweep_code_row(Z), n_words, kind == TF_KIND_SOLVER ? 3 : 0);
     TF_LAUNCH_CHECK("k_pack_sweep_codes");
     return 0;
 }

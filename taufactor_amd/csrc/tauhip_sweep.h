@@ -868,10 +868,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                 const float yp = j < RPW - 1 ? (a ? hi_f(F[sl][j + 1]) : lo_f(F[sl][j + 1])) : ypL;
                 const float zp = a ? from_next_lane(m.x) : hi_f(m);
                 const float zm = a ? lo_f(m) : from_prev_lane(m.y);
+                // Solver: the byte is 8 x the count, the offset of its table entry
                 const uint32_t code = a ? (K[sl][j] >> 8) : (K[sl][j] & 0xffu);
                 float out;
                 if constexpr (KIND == TF_KIND_SOLVER) {
-                    const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
+                    const float2 rdv =
+                        *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(sT) + code);
                     float s = xp + xm;
                     s = s + yp;
                     s = s + ym;
@@ -881,7 +883,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     const float e2 = __builtin_fmaf(-qt, rdv.y, s);
                     qt = __builtin_fmaf(e2, rdv.x, qt);
                     if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
-                        qt = s / (code ? (float)code : __builtin_inff());
+                        qt = s / (code ? (float)(code >> 3) : __builtin_inff());
                     s = qt - c;
                     s = s * w;
                     out = c + s;

@@ -168,7 +168,7 @@ class SORSolver(ABC):
         """Repack the codes into the aligned layout the sweep reads."""
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
         st = self._lib.tf_pack_sweep_codes(_lib.ptr(self._codes), _lib.ptr(self._sweep_codes),
-                                           B, X, Y, Z, self._stream())
+                                           self.kind, B, X, Y, Z, self._stream())
         _lib.check(st, "tf_pack_sweep_codes")
 
     _tunable = True

@@ -105,14 +105,15 @@ def test_multiphase_label_zero_conductive_and_odd_values():
 
 @pytest.mark.parametrize("shape", [(3, 4, 5), (2, 3, 12), (4, 2, 1)])
 def test_sweep_code_layout(shape):
-    """tf_pack_sweep_codes: byte z+3 of each row holds code z, the rest 0."""
+    """tf_pack_sweep_codes: byte z+3 of each row holds code z (times 8 for the
+    Solver kind: the byte offset of its table entry), the rest 0."""
     img = _binary(shape, seed=4)
     s = tau.Solver(np.stack([img, 1 - img]))
     X, Y, Z = shape
     row = ((Z + 5 + 3) // 4) * 4
     got = s._sweep_codes.cpu().numpy().reshape(2, X + 2, Y + 2, row)
     want = np.zeros_like(got)
-    want[..., 3:Z + 5] = s._codes.cpu().numpy()
+    want[..., 3:Z + 5] = s._codes.cpu().numpy() * 8
     assert np.array_equal(got, want)
 
 
