@@ -174,6 +174,11 @@ __device__ __forceinline__ float update_multiphase(const Nbr& n, float gxp, floa
 // ---------------------------------------------------------------------------
 // temporally blocked sweep: H half-sweeps per HBM pass
 // ---------------------------------------------------------------------------
+// Two kernels share this scheme: rsweep_tile (register-blocked; Solver,
+// anisotropic, electrode, per-voxel field) and sweep_tile below (LDS ring;
+// MultiPhase, whose update reads every neighbour's class).  The rest of this
+// comment describes sweep_tile.
+//
 // A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
 // coordinates) and streams it plane by plane along x through a ring of H+3
 // planes in LDS, each plane extended by a halo in y and z.  Half-sweep h
@@ -257,10 +262,6 @@ struct SweepShape {
     static constexpr int RPL = RPW / 2;      // row pairs loaded per wave and plane
 };
 
-// per-voxel auxiliary data: a one-byte code (Solver, MultiPhase) or the
-// voxel diffusivity (per-voxel field)
-template <int KIND>
-using code_t = std::conditional_t<KIND == TF_KIND_FIELD, float, uint8_t>;
 
 // 2ab/(a+b), 0 when a+b == 0, with the reference's float32 operation order
 // ((2*a)*b)/(a+b) (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
@@ -269,6 +270,11 @@ __device__ __forceinline__ float hmean(float a, float b) {
     return den > 0.0f ? ((2.0f * a) * b) / den : 0.0f;
 }
 
+// per-voxel auxiliary data the layer sums read: a one-byte code (Solver,
+// MultiPhase, anisotropic, electrode) or the voxel diffusivity (field)
+template <int KIND>
+using code_t = std::conditional_t<KIND == TF_KIND_FIELD, float, uint8_t>;
+
 // (1/d, d) per Solver code for the sweep.  Code 0 (factor inf) uses (0, 0):
 // the sequence of div_by_count then yields a zero whose sign can differ from
 // s/inf only when s is -0, and c + w*(q - c) is the same float for either
@@ -276,13 +282,13 @@ __device__ __forceinline__ float hmean(float a, float b) {
 static __constant__ float c_rd[18] = {0.0f, 0.0f, 1.0f, 1.0f, 0.5f, 2.0f, 1.0f / 3.0f, 3.0f, 0.25f, 4.0f,
                                0.2f, 5.0f, 1.0f / 6.0f, 6.0f, 1.0f / 7.0f, 7.0f, 0.125f, 8.0f};
 
-template <int H, int KIND, int NW, int RPW>
+template <int H, int NW, int RPW>
 __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
-                                           const code_t<KIND>* __restrict__ Cg,
+                                           const uint8_t* __restrict__ Cg,
                                            const float* __restrict__ hm, int ncls, const Vol v,
                                            float w, int colour0, int TX, int tz, int ty, int tx) {
     using Sh = SweepShape<H, NW, RPW>;
-    using CT = code_t<KIND>;
+    using CT = uint8_t;
     constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NP = Sh::NP, PS = Sh::PS, RPL = Sh::RPL;
     static_assert(TY >= 1, "tile has no interior rows");
     static_assert(NW % 2 == 0, "the rows of a wave must share their colour parity");
@@ -301,22 +307,11 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     const int zl = ew + 2 * lane;   // z of the first voxel of this lane's pair
     const int lx1 = min(x1 + H, v.X + 2);
 
-    if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
-    } else if (KIND == TF_KIND_MULTIPHASE) {
-        for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
-    } else if (KIND == TF_KIND_ANISOTROPIC) {
-        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    } else if (KIND == TF_KIND_ELECTRODE) {
-        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
-    }
-    // spacing weights of TF_KIND_ANISOTROPIC (wave-uniform)
-    const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
-    const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
+    for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
 
     // ---- global memory ----
-    // Loads: one descriptor per array for the whole tile.  The field (and
-    // the per-voxel diffusivity) descriptor starts kSlackBytes before the
+    // Loads: one descriptor per array for the whole tile.  The field
+    // descriptor starts kSlackBytes before the
     // volume and spans it plus the slack on both sides, so the window's
     // first quad (z = ew >= -3) is never at a negative offset; the lane
     // offset holds row and z, the wave-uniform offset the plane.  Rows
@@ -337,30 +332,19 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
     const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
                                 (uint32_t)vol_f + 2u * kSlackBytes);
     int vf[RPL], vc[RPL];
-    rsrc_t rC;
-    int plane_c;
-    if constexpr (KIND == TF_KIND_FIELD) {
-        plane_c = plane_f;
-        rC = make_rsrc(reinterpret_cast<const char*>(Cg) - kSlackBytes,
-                       (uint32_t)vol_f + 2u * kSlackBytes);
-    } else {
-        const int row_c = sweep_code_row(v.Z);
-        plane_c = (v.Y + 2) * row_c;
-        rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
-    }
+    const int row_c = sweep_code_row(v.Z);
+    const int plane_c = (v.Y + 2) * row_c;
+    const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
 #pragma unroll
     for (int i = 0; i < RPL; ++i) {
         const int y = ey0 + 2 * (wave + i * NW) + half;
         const bool rok = y >= 0 && y <= v.Y + 1;
         vf[i] = rok ? kSlackBytes + (y * v.sY + ew + 4 * qd) * 4 : kOff;
-        if constexpr (KIND == TF_KIND_FIELD)
-            vc[i] = vf[i];
-        else
-            vc[i] = rok ? y * sweep_code_row(v.Z) + (ew + 3) + 4 * qd : kOff;
+        vc[i] = rok ? y * row_c + (ew + 3) + 4 * qd : kOff;
     }
 
     u32x4_t pf[NP][RPL];
-    std::conditional_t<KIND == TF_KIND_FIELD, u32x4_t, uint32_t> pc[NP][RPL];
+    uint32_t pc[NP][RPL];
     auto issue_load = [&](int p, auto reg_c) {
         constexpr int rg = decltype(reg_c)::value;
         const bool pok = p < lx1;
@@ -368,10 +352,7 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
 #pragma unroll
         for (int i = 0; i < RPL; ++i) {
             pf[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rF, pok ? vf[i] : kOff, sf, 0);
-            if constexpr (KIND == TF_KIND_FIELD)
-                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rC, pok ? vc[i] : kOff, sc, 0);
-            else
-                pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b32(rC, pok ? vc[i] : kOff, sc, 0);
+            pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b32(rC, pok ? vc[i] : kOff, sc, 0);
         }
     };
     // this lane's first quad slot (pairs 2qd, 2qd+1) in row 2*wave+half of slot 0
@@ -385,17 +366,10 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
             const u32x4_t q = pf[rg][i];
             *reinterpret_cast<u32x2_t*>(Fl + o) = u32x2_t{q.x, q.z};               // first voxels
             *reinterpret_cast<u32x2_t*>(Fl + o + kRowPairs) = u32x2_t{q.y, q.w};   // second voxels
-            if constexpr (KIND == TF_KIND_FIELD) {
-                const u32x4_t k = pc[rg][i];
-                *reinterpret_cast<u32x2_t*>(Cl + o) = u32x2_t{k.x, k.z};
-                *reinterpret_cast<u32x2_t*>(Cl + o + kRowPairs) = u32x2_t{k.y, k.w};
-            } else {
-                const uint32_t c = pc[rg][i];  // codes of z = 4qd+ew .. +3, byte 0 first
-                *reinterpret_cast<uint16_t*>(Cl + o) =
-                    (uint16_t)((c & 0xffu) | ((c >> 8) & 0xff00u));
-                *reinterpret_cast<uint16_t*>(Cl + o + kRowPairs) =
-                    (uint16_t)(((c >> 8) & 0xffu) | ((c >> 16) & 0xff00u));
-            }
+            const uint32_t c = pc[rg][i];  // codes of z = 4qd+ew .. +3, byte 0 first
+            *reinterpret_cast<uint16_t*>(Cl + o) = (uint16_t)((c & 0xffu) | ((c >> 8) & 0xff00u));
+            *reinterpret_cast<uint16_t*>(Cl + o + kRowPairs) =
+                (uint16_t)(((c >> 8) & 0xffu) | ((c >> 16) & 0xff00u));
         }
     };
     // stores: rows of the compute mapping, one pair per lane
@@ -495,64 +469,10 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
             n.yp = Fq[o + kRowLen];
             n.zp = Fp[q][o];
             n.zm = Fm[q][o];
-            float out;
-            if constexpr (KIND == TF_KIND_SOLVER) {
-                const int code = Co[q][o];
-                const float2 rdv = reinterpret_cast<const float2*>(sT)[code];
-                float s = n.xp + n.xm;
-                s = s + n.yp;
-                s = s + n.ym;
-                s = s + n.zp;
-                s = s + n.zm;
-                float qt = s * rdv.x;
-                const float e = __builtin_fmaf(-qt, rdv.y, s);
-                qt = __builtin_fmaf(e, rdv.x, qt);
-                // below 2^-100 the residual correction can be off (d = 6):
-                // IEEE division there (s = +-0 needs none: any zero quotient
-                // gives the same update)
-                if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
-                    qt = s / (code ? (float)code : __builtin_inff());
-                s = qt - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_ELECTRODE) {
-                // SORSolver.sum_weighted_neighbours / factor with the reaction
-                // term in the factor (taufactor/taufactor.py:47-56, 95-103, 175-181)
-                float s = n.xp + n.xm;
-                s = s + n.yp;
-                s = s + n.ym;
-                s = s + n.zp;
-                s = s + n.zm;
-                s = s / sT[Co[q][o]];
-                s = s - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
-                // ((x+ + x-) + Ky*(y+ + y-)) + Kz*(z+ + z-), / nn
-                // (AnisotropicSolver.sum_weighted_neighbours, taufactor/taufactor.py:473-478)
-                float s = n.xp + n.xm;
-                float t = n.yp + n.ym;
-                t = ky * t;
-                s = s + t;
-                t = n.zp + n.zm;
-                t = kz * t;
-                s = s + t;
-                s = s / sT[Co[q][o]];
-                s = s - n.c;
-                s = s * w;
-                out = n.c + s;
-            } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
-                const float* row = sT + (int)Co[q][o] * ncls;
-                out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]], row[Co[q][o + kRowLen]],
-                                        row[Co[q][o - kRowLen]], row[Cp[q][o]], row[Cm[q][o]],
-                                        p == 1, p == v.X, w);
-            } else {
-                const float kc = Co[q][o];
-                out = update_multiphase(n, hmean(kc, Co[q][op]), hmean(kc, Co[q][om]),
-                                        hmean(kc, Co[q][o + kRowLen]), hmean(kc, Co[q][o - kRowLen]),
-                                        hmean(kc, Cp[q][o]), hmean(kc, Cm[q][o]), p == 1, p == v.X,
-                                        w);
-            }
+            const float* row = sT + (int)Co[q][o] * ncls;
+            const float out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]],
+                                                row[Co[q][o + kRowLen]], row[Co[q][o - kRowLen]],
+                                                row[Cp[q][o]], row[Cm[q][o]], p == 1, p == v.X, w);
             Fq[o] = out;
         }
     };
@@ -982,20 +902,20 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     tx = t % a.nTx;
     b = t / a.nTx;
     const size_t off = (size_t)b * a.v.sB;
-    const size_t coff = KIND == TF_KIND_FIELD ? off
-                                              : (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
+    // per-volume codes: the sweep-code layout, or for the field kind the
+    // face sections (floats in the padded field layout)
+    const uint8_t* codes =
+        KIND == TF_KIND_FIELD
+            ? reinterpret_cast<const uint8_t*>(static_cast<const float*>(a.codes) + off)
+            : static_cast<const uint8_t*>(a.codes) + (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
     if constexpr (reg_kind(KIND))
-        rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                      reinterpret_cast<const uint8_t*>(
-                                          static_cast<const code_t<KIND>*>(a.codes) + coff),
-                                      hm, a.v, a.w, a.colour0, a.TX,
+        rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
                                       nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx,
                                       nar, a.fsec);
     else
-        sweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off,
-                                     static_cast<const code_t<KIND>*>(a.codes) + coff, hm, a.ncls,
-                                     a.v, a.w, a.colour0, a.TX, tz, ty, tx);
+        sweep_tile<H, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.ncls, a.v, a.w, a.colour0,
+                               a.TX, tz, ty, tx);
 }
 
 // Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows
@@ -1011,14 +931,9 @@ struct SweepGeom {
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
     if (reg_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes(kind);
+    // LDS ring (MultiPhase): field and class planes, then the face table
     using Sh = SweepShape<H, NW, RPW>;
-    const size_t code_bytes = kind == TF_KIND_FIELD ? sizeof(float) : 1;
-    size_t bytes = kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + code_bytes);
-    bytes += kind == TF_KIND_MULTIPHASE    ? (size_t)ncls * ncls * sizeof(float)
-             : kind == TF_KIND_ANISOTROPIC ? TF_ANISO_TABLE * sizeof(float)
-             : kind == TF_KIND_ELECTRODE   ? TF_ELECTRODE_TABLE * sizeof(float)
-                                           : 18 * sizeof(float);
-    return bytes;
+    return kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + 1) + (size_t)ncls * ncls * sizeof(float);
 }
 
 // Workgroups of one sweep kernel resident on the current device at once
