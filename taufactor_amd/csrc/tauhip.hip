@@ -572,7 +572,11 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
                                                             const float* __restrict__ hm, int ncls,
                                                             Vol v, double* __restrict__ flux,
                                                             double* __restrict__ conc) {
-    const int x = blockIdx.x, b = blockIdx.y;
+    // XCD-aware numbering (gridDim.x is a multiple of 8): blocks x and x+1,
+    // which both read layer x+1, run on the same XCD and share its L2
+    const int per = gridDim.x >> 3;
+    const int x = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3), b = blockIdx.y;
+    if (x >= v.X) return;
     double sc = 0.0, sf = 0.0;
     layer_sum_block<KIND>(F + (size_t)b * v.sB, C + (size_t)b * v.sB, hm, ncls, v, x, sc, sf);
     if (threadIdx.x == 0) {
@@ -594,7 +598,7 @@ extern "C" int tf_layer_sums(const float* field, const void* codes, const float*
     if (X > 65535 || B > 65535) return fail(-1, "grid too large for layer sums");
     Vol v = make_vol(X, Y, Z);
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid(X, B);
+    dim3 grid((X + 7) / 8 * 8, B);
     if (kind == TF_KIND_SOLVER)
         hipLaunchKernelGGL(k_layer_sums<TF_KIND_SOLVER>, grid, dim3(kSumThreads), 0, st, field,
                            static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
