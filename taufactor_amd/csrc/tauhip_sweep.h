@@ -174,40 +174,29 @@ __device__ __forceinline__ float update_multiphase(const Nbr& n, float gxp, floa
 // ---------------------------------------------------------------------------
 // temporally blocked sweep: H half-sweeps per HBM pass
 // ---------------------------------------------------------------------------
-// Two kernels share this scheme: rsweep_tile (register-blocked; Solver,
-// anisotropic, electrode, per-voxel field) and sweep_tile below (LDS ring;
-// MultiPhase, whose update reads every neighbour's class).  The rest of this
-// comment describes sweep_tile.
-//
 // A workgroup owns an interior tile [y0,y1) x [z0,z1) x [x0,x1) (padded
-// coordinates) and streams it plane by plane along x through a ring of H+3
-// planes in LDS, each plane extended by a halo in y and z.  Half-sweep h
-// (h = 0..H-1, colour first_colour^(h&1)) runs on plane s-h at step s, on
-// the tile grown by H-1-h voxels, so that after the last half-sweep the
-// owned tile holds exactly the values of H successive reference iterations.
-// Red-black colouring makes each half-sweep a Jacobi step within one colour,
-// so reading src and writing dst (no in-place hazard between workgroups
-// that recompute each other's halos) gives the in-place result.
+// coordinates) and streams it plane by plane along x, each plane extended by
+// a halo in y and z.  Half-sweep h (h = 0..H-1, colour first_colour^(h&1))
+// runs on plane s-h at step s, on the tile grown by H-1-h voxels, so that
+// after the last half-sweep the owned tile holds exactly the values of H
+// successive reference iterations.  Red-black colouring makes each
+// half-sweep a Jacobi step within one colour, so reading src and writing dst
+// (no in-place hazard between workgroups that recompute each other's halos)
+// gives the in-place result.  The kernel is rsweep_tile below.
 //
-// Compute mapping: one wavefront per tile row (y), one lane per pair of
-// voxels in z, so an LDS row is a window of 128 voxels of z starting at ew
-// and the colour pattern of a row is uniform across the wave.  An LDS row
-// stores the first voxel of each pair (64), then the second (64): every LDS
-// access of a half-sweep is lane-contiguous (no bank conflicts).  Each wave
-// owns RPW rows (interleaved, NW even so that all rows of a wave share their
-// colour parity), NW waves make EY = NW*RPW rows (tile interior TY = EY-2H).
-// The window starts ML >= H voxels before the tile, with ML and the tile
-// width TZ multiples of 4 so that z0 and ew are 1 mod 4.
-//
-// Memory mapping: a plane of the tile is read with one 16-byte load per lane
-// covering two rows (lanes 0-31: row 2m, lanes 32-63: row 2m+1, four z
-// each), codes with one 4-byte load per lane from the aligned sweep-code
-// layout, and written back with one 8-byte store per lane and row (one pair
-// each).  The x loop is unrolled by the ring length NP, and plane q lives in
-// LDS slot and register slot (q - S0 + 1) mod NP: every slot is a
-// compile-time constant, so each LDS access is one instruction with an
-// immediate offset from one of a few per-lane base addresses.  Planes are
-// loaded NP-1 steps ahead of their use.
+// Global memory conventions.  Loads: one descriptor per array for the whole
+// tile.  The field descriptor starts kSlackBytes before the volume and
+// spans it plus the slack on both sides, so the window's first pair (z = ew,
+// at least 1 - 6) is at a non-negative offset whenever its row exists; the
+// lane offset holds row and z, the wave-uniform offset the plane.  Rows
+// outside [0, Y+1] and planes outside the tile's stream use a lane offset
+// past any record count: loads read 0, no memory traffic.  The range check
+// is per dword, so pairs running past the volume read 0 there.  Stores: one
+// descriptor per row and plane, based at z = 0 of that row with record
+// count 4*z1 (0 outside the owned rows and planes), so a pair straddling
+// z1 = Z+1 stores only its first voxel; pairs before z0 use the
+// out-of-range lane offset.  No branches around memory instructions, so the
+// compiler's waitcnt pass keeps the prefetched planes in flight.
 //
 // Workgroups are numbered so that consecutive tiles land on the same XCD
 // (hardware dispatch sends workgroup i to XCD i mod 8): neighbouring tiles
@@ -243,24 +232,11 @@ template <int I>
 using ic = std::integral_constant<int, I>;
 
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kRowPairs = 64;           // lanes per row
 constexpr int kRowLen = 2 * kRowPairs;  // voxels per LDS row
-constexpr int kLdsPad = kRowLen * 4;    // bytes before the plane ring: keeps the row-below
-                                        // bias of the base addresses non-negative
 constexpr int kSlackBytes = TF_FIELD_SLACK_BYTES;
 
-template <int H, int NW, int RPW>
-struct SweepShape {
-    static constexpr int EY = NW * RPW;
-    static constexpr int TY = EY - 2 * H;
-    static constexpr int ML = 4 * ((H + 3) / 4);                // z0 - ew
-    static constexpr int TZ = ((kRowLen - ML - H) / 4) * 4;     // owned voxels of z
-    static constexpr int NP = H + 3;         // planes in the LDS ring and in the register ring
-    static constexpr int PS = EY * kRowLen;  // floats (and codes) per plane
-    static constexpr int RPL = RPW / 2;      // row pairs loaded per wave and plane
-};
 
 
 // 2ab/(a+b), 0 when a+b == 0, with the reference's float32 operation order
@@ -282,244 +258,13 @@ using code_t = std::conditional_t<KIND == TF_KIND_FIELD, float, uint8_t>;
 static __constant__ float c_rd[18] = {0.0f, 0.0f, 1.0f, 1.0f, 0.5f, 2.0f, 1.0f / 3.0f, 3.0f, 0.25f, 4.0f,
                                0.2f, 5.0f, 1.0f / 6.0f, 6.0f, 1.0f / 7.0f, 7.0f, 0.125f, 8.0f};
 
-template <int H, int NW, int RPW>
-__device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* __restrict__ D,
-                                           const uint8_t* __restrict__ Cg,
-                                           const float* __restrict__ hm, int ncls, const Vol v,
-                                           float w, int colour0, int TX, int tz, int ty, int tx) {
-    using Sh = SweepShape<H, NW, RPW>;
-    using CT = uint8_t;
-    constexpr int TY = Sh::TY, TZ = Sh::TZ, ML = Sh::ML, NP = Sh::NP, PS = Sh::PS, RPL = Sh::RPL;
-    static_assert(TY >= 1, "tile has no interior rows");
-    static_assert(NW % 2 == 0, "the rows of a wave must share their colour parity");
-    static_assert(RPW % 2 == 0, "rows are loaded in pairs");
-    extern __shared__ __align__(16) unsigned char smem[];
-    float* sF = reinterpret_cast<float*>(smem + kLdsPad);
-    CT* sC = reinterpret_cast<CT*>(sF + NP * PS);
-    float* sT = reinterpret_cast<float*>(sC + NP * PS);
-
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int y0 = 1 + ty * TY, z0 = 1 + tz * TZ, x0 = 1 + tx * TX;
-    const int y1 = min(y0 + TY, v.Y + 1), z1 = min(z0 + TZ, v.Z + 1), x1 = min(x0 + TX, v.X + 1);
-    const int ey0 = y0 - H;
-    const int ew = z0 - ML;         // z of window element 0; ew = 1 mod 4
-    const int zl = ew + 2 * lane;   // z of the first voxel of this lane's pair
-    const int lx1 = min(x1 + H, v.X + 2);
-
-    for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
-
-    // ---- global memory ----
-    // Loads: one descriptor per array for the whole tile.  The field
-    // descriptor starts kSlackBytes before the
-    // volume and spans it plus the slack on both sides, so the window's
-    // first quad (z = ew >= -3) is never at a negative offset; the lane
-    // offset holds row and z, the wave-uniform offset the plane.  Rows
-    // outside [0, Y+1] and planes past the tile's last plane use a lane
-    // offset past any record count: loads read 0, no memory traffic.  The
-    // range check is per dword, so quads running past the volume read 0
-    // there.
-    // Stores: one descriptor per row and plane, based at z = 0 of that row
-    // with record count 4*z1 (0 outside the owned rows and planes), so a
-    // pair straddling z1 = Z+1 stores only its first voxel; pairs before z0
-    // use the out-of-range lane offset.
-    // No branches around memory instructions, so the waitcnt pass keeps NP-1
-    // planes in flight.
-    constexpr int kOff = 0x7fffff00;  // lane offset past any record count
-    const int plane_f = v.sX * 4;
-    const int vol_f = (int)(v.sB * 4);
-    const int half = lane >> 5, qd = lane & 31;
-    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
-                                (uint32_t)vol_f + 2u * kSlackBytes);
-    int vf[RPL], vc[RPL];
-    const int row_c = sweep_code_row(v.Z);
-    const int plane_c = (v.Y + 2) * row_c;
-    const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
-#pragma unroll
-    for (int i = 0; i < RPL; ++i) {
-        const int y = ey0 + 2 * (wave + i * NW) + half;
-        const bool rok = y >= 0 && y <= v.Y + 1;
-        vf[i] = rok ? kSlackBytes + (y * v.sY + ew + 4 * qd) * 4 : kOff;
-        vc[i] = rok ? y * row_c + (ew + 3) + 4 * qd : kOff;
-    }
-
-    u32x4_t pf[NP][RPL];
-    uint32_t pc[NP][RPL];
-    auto issue_load = [&](int p, auto reg_c) {
-        constexpr int rg = decltype(reg_c)::value;
-        const bool pok = p < lx1;
-        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
-#pragma unroll
-        for (int i = 0; i < RPL; ++i) {
-            pf[rg][i] = __builtin_amdgcn_raw_buffer_load_b128(rF, pok ? vf[i] : kOff, sf, 0);
-            pc[rg][i] = __builtin_amdgcn_raw_buffer_load_b32(rC, pok ? vc[i] : kOff, sc, 0);
-        }
-    };
-    // this lane's first quad slot (pairs 2qd, 2qd+1) in row 2*wave+half of slot 0
-    float* const Fl = sF + (2 * wave + half) * kRowLen + 2 * qd;
-    CT* const Cl = sC + (2 * wave + half) * kRowLen + 2 * qd;
-    auto commit_load = [&](auto slot_c, auto reg_c) {
-        constexpr int sl = decltype(slot_c)::value, rg = decltype(reg_c)::value;
-#pragma unroll
-        for (int i = 0; i < RPL; ++i) {
-            const int o = sl * PS + 2 * i * NW * kRowLen;
-            const u32x4_t q = pf[rg][i];
-            *reinterpret_cast<u32x2_t*>(Fl + o) = u32x2_t{q.x, q.z};               // first voxels
-            *reinterpret_cast<u32x2_t*>(Fl + o + kRowPairs) = u32x2_t{q.y, q.w};   // second voxels
-            const uint32_t c = pc[rg][i];  // codes of z = 4qd+ew .. +3, byte 0 first
-            *reinterpret_cast<uint16_t*>(Cl + o) = (uint16_t)((c & 0xffu) | ((c >> 8) & 0xff00u));
-            *reinterpret_cast<uint16_t*>(Cl + o + kRowPairs) =
-                (uint16_t)(((c >> 8) & 0xffu) | ((c >> 16) & 0xff00u));
-        }
-    };
-    // stores: rows of the compute mapping, one pair per lane
-    const float* Fw = sF + wave * kRowLen + lane;
-    const int zs = zl >= z0 ? zl * 4 : kOff;
-    bool own_row[RPW];
-    int ys[RPW];
-#pragma unroll
-    for (int j = 0; j < RPW; ++j) {
-        const int y = ey0 + wave + j * NW;
-        own_row[j] = y >= y0 && y < y1;
-        ys[j] = own_row[j] ? y : 0;
-    }
-    auto store_plane = [&](int q, auto slot_c) {
-        constexpr int sl = decltype(slot_c)::value;
-        const bool qok = q >= x0 && q < x1;
-        const int qs = qok ? q : 0;
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            const int o = sl * PS + j * NW * kRowLen;
-            const rsrc_t r = make_rsrc(D + qs * v.sX + ys[j] * v.sY,
-                                       (qok && own_row[j]) ? (uint32_t)(4 * z1) : 0u);
-            const u32x2_t val{__builtin_bit_cast(uint32_t, Fw[o]),
-                              __builtin_bit_cast(uint32_t, Fw[o + kRowPairs])};
-            __builtin_amdgcn_raw_buffer_store_b64(val, r, zs, 0, 0);
-        }
-    };
-
-    // ---- per-tile bounds of each half-sweep ----
-    // planes [plo[h], plo[h] + pn[h]) and, as bit h*RPW+j of rowbits, the
-    // active rows.  In z every lane updates its active element: outside the
-    // half-sweep's region the result is never read by a later half-sweep of
-    // this pass nor stored, and on the z ghosts (value 0, code 0 / class 0 /
-    // k 0) the update returns the ghost's value.
-    int plo[H], pn[H];
-    int rowbits = 0;
-#pragma unroll
-    for (int h = 0; h < H; ++h) {
-        const int grow = H - 1 - h;
-        plo[h] = max(x0 - grow, 1);
-        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
-        const int ry0 = max(y0 - grow, 1), ry1 = min(y1 + grow, v.Y + 1);
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            const int y = ey0 + wave + j * NW;
-            rowbits |= (y >= ry0 && y < ry1) ? 1 << (h * RPW + j) : 0;
-        }
-    }
-
-    // The colour of (p, y, z) is (p+y+z+1)&1 and half-sweep h of step s
-    // updates colour colour0^(h&1) on plane s-h, so the active element of a
-    // lane's pair, act = (s + y + ew + 1 + colour0) & 1, is the same for all
-    // half-sweeps of a step and all rows of a wave; it alternates with s.
-    const int S0 = max(x0 - (H - 1), 1);
-    const int S1 = (x1 - 1) + (H - 1);
-    int A = (S0 + ey0 + wave + ew + 1 + colour0) & 1;  // act of the even steps of an iteration
-
-    // base addresses of the own element and its z neighbours for either
-    // parity of the step, biased one row down (the row below is at offset 0)
-    const float* Fo[2];
-    const float* Fp[2];
-    const float* Fm[2];
-    const CT* Co[2];
-    const CT* Cp[2];
-    const CT* Cm[2];
-    auto set_bases = [&](int a0) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int a = a0 ^ q;
-            const int r0 = wave * kRowLen - kRowLen + lane;
-            const int io = r0 + (a ? kRowPairs : 0);
-            const int ip = r0 + (a ? 1 : kRowPairs);      // z+1: next pair's first / own second
-            const int im = r0 + (a ? 0 : kRowPairs - 1);  // z-1: own first / previous pair's second
-            Fo[q] = sF + io; Fp[q] = sF + ip; Fm[q] = sF + im;
-            Co[q] = sC + io; Cp[q] = sC + ip; Cm[q] = sC + im;
-        }
-    };
-
-    // ---- one half-sweep of one plane ----
-    auto stage = [&](auto h_c, auto q_c, auto sp_c, int p) {
-        constexpr int h = decltype(h_c)::value, q = decltype(q_c)::value;
-        constexpr int sp = decltype(sp_c)::value;
-        constexpr int spm = (sp + NP - 1) % NP, spp = (sp + 1) % NP;
-        if ((unsigned)(p - plo[h]) >= (unsigned)pn[h]) return;
-        float* const Fq = const_cast<float*>(Fo[q]);
-#pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-            if (!((rowbits >> (h * RPW + j)) & 1)) continue;
-            const int o = sp * PS + j * NW * kRowLen + kRowLen;
-            const int om = spm * PS + j * NW * kRowLen + kRowLen;
-            const int op = spp * PS + j * NW * kRowLen + kRowLen;
-            Nbr n;
-            n.c = Fq[o];
-            n.xm = Fq[om];
-            n.xp = Fq[op];
-            n.ym = Fq[o - kRowLen];
-            n.yp = Fq[o + kRowLen];
-            n.zp = Fp[q][o];
-            n.zm = Fm[q][o];
-            const float* row = sT + (int)Co[q][o] * ncls;
-            const float out = update_multiphase(n, row[Co[q][op]], row[Co[q][om]],
-                                                row[Co[q][o + kRowLen]], row[Co[q][o - kRowLen]],
-                                                row[Cp[q][o]], row[Cm[q][o]], p == 1, p == v.X, w);
-            Fq[o] = out;
-        }
-    };
-
-    // planes S0-1 .. S0+1 go straight to LDS (slots 0..2); S0+2 .. S0+NP are put in flight
-    issue_load(S0 - 1, ic<0>{});
-    commit_load(ic<0>{}, ic<0>{});
-    issue_load(S0, ic<1>{});
-    commit_load(ic<1>{}, ic<1>{});
-    issue_load(S0 + 1, ic<2>{});
-    commit_load(ic<2>{}, ic<2>{});
-    static_for<2, NP + 1>([&](auto d_c) {
-        constexpr int d = decltype(d_c)::value;
-        issue_load(S0 + d, ic<(d + 1) % NP>{});
-    });
-    __syncthreads();
-
-    // step s = s0 + u: put plane s+1+NP in flight, run half-sweep h on plane
-    // s-h, write plane s-H+1 out, move plane s+2 from registers to LDS.
-    // Steps past S1 (to complete the last iteration) do no work but keep the
-    // memory instructions unconditional.
-    for (int s0 = S0; s0 <= S1; s0 += NP) {
-        set_bases(A);
-        static_for<0, NP>([&](auto u_c) {
-            constexpr int u = decltype(u_c)::value;
-            const int s = s0 + u;
-            issue_load(s + 1 + NP, ic<(u + 2) % NP>{});
-            static_for<0, H>([&](auto h_c) {
-                constexpr int h = decltype(h_c)::value;
-                stage(h_c, ic<(u & 1)>{}, ic<((u + 1 - h) % NP + NP) % NP>{}, s - h);
-                if constexpr (h + 1 < H) __syncthreads();
-            });
-            store_plane(s - (H - 1), ic<((u + 2 - H) % NP + NP) % NP>{});
-            commit_load(ic<(u + 3) % NP>{}, ic<(u + 3) % NP>{});
-            __syncthreads();
-        });
-        A ^= NP & 1;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// register-blocked sweep for the kinds whose update reads only the voxel's
-// own code (Solver, anisotropic, electrode)
+// the register-blocked sweep (every kind)
 // ---------------------------------------------------------------------------
-// Same compute mapping as sweep_tile (a lane holds the z-pair zl, zl+1 of
-// each of its rows; the red/black pattern is wave-uniform), but a wave owns
-// RPW consecutive rows and keeps every plane of the ring in registers:
+// Compute mapping: one lane per pair of voxels in z (a 128-voxel window of z
+// starting at ew per row), so the red/black pattern is uniform across a
+// wave.  A wave owns RPW consecutive rows and keeps every plane of the ring
+// in registers:
 // x neighbours and the y neighbours inside the wave are registers, the z
 // neighbour across lanes comes by a DPP wave shift, and only the first and
 // last row of each wave go through LDS.  Half-sweep h of step s reads the
@@ -529,12 +274,7 @@ __device__ __forceinline__ void sweep_tile(const float* __restrict__ S, float* _
 __host__ __device__ constexpr bool own_code_kind(int kind) {
     return kind == TF_KIND_SOLVER || kind == TF_KIND_ANISOTROPIC || kind == TF_KIND_ELECTRODE;
 }
-// kinds swept by the register-blocked kernel: the own-code kinds and the
-// per-voxel field kind, whose face conductances are precomputed
-// (tf_field_faces)
-__host__ __device__ constexpr bool reg_kind(int kind) {
-    return own_code_kind(kind) || kind == TF_KIND_FIELD;
-}
+
 
 template <int H, int NW, int RPW>
 struct RegShape {
@@ -551,9 +291,18 @@ struct RegShape {
     static constexpr int EB = (NW + 2) * 2 * kRowPairs;       // floats per edge buffer
     // field kind: the y faces of each wave's last row, per ring plane
     static constexpr int GEB = (NW + 2) * 2 * kRowPairs;     // floats per ring plane
-    static constexpr size_t lds_bytes(int kind) {
-        return TBL + (size_t)H * 2 * EB * sizeof(float) +
-               (kind == TF_KIND_FIELD ? (size_t)NR * GEB * sizeof(float) : 0);
+    // MultiPhase kind: the class codes (16 bits: the lane's pair) of each
+    // wave's first and last row, per ring plane
+    static constexpr int KEB = (NW + 2) * 2 * kRowPairs;     // uint16 per ring plane
+    // bytes before the edge buffers: the (1/d, d) or prefactor table, or the
+    // ncls x ncls face-conductance table of the MultiPhase kind
+    static constexpr int table_bytes(int kind, int ncls) {
+        return kind == TF_KIND_MULTIPHASE ? std::max(TBL, (ncls * ncls * 4 + 15) / 16 * 16) : TBL;
+    }
+    static constexpr size_t lds_bytes(int kind, int ncls = 0) {
+        return table_bytes(kind, ncls) + (size_t)H * 2 * EB * sizeof(float) +
+               (kind == TF_KIND_FIELD ? (size_t)NR * GEB * sizeof(float) : 0) +
+               (kind == TF_KIND_MULTIPHASE ? (size_t)NR * KEB * sizeof(uint16_t) : 0);
     }
 };
 
@@ -591,20 +340,24 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
                                             int colour0, int TX, int z0, int ty, int tx, bool nar,
-                                            long long fsec = 0) {
+                                            long long fsec = 0, int ncls = 0) {
     using Sh = RegShape<H, NW, RPW>;
     // field kind: Cg holds three sections of face conductances in the padded
     // field layout, fsec floats apart: gx, gy, gz, each stored at the voxel
     // on the face's minus side (tf_field_faces)
     constexpr bool kFaces = KIND == TF_KIND_FIELD;
+    // MultiPhase kind: Cg holds the class codes (sweep-code layout), hm the
+    // ncls x ncls face-conductance table
+    constexpr bool kClasses = KIND == TF_KIND_MULTIPHASE;
     constexpr int NG = kFaces ? Sh::NR : 1;
     constexpr int TY = Sh::TY, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
     static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
     static_assert(RPW % 2 == 0, "rows per wave must be even");
     extern __shared__ __align__(16) unsigned char smem[];
     float* const sT = reinterpret_cast<float*>(smem);
-    float* const sE = reinterpret_cast<float*>(smem + Sh::TBL);
+    float* const sE = reinterpret_cast<float*>(smem + Sh::table_bytes(KIND, ncls));
     float* const sGE = sE + (size_t)H * 2 * EB;  // field kind: y faces of edge rows
+    uint16_t* const sKE = reinterpret_cast<uint16_t*>(sGE);  // MultiPhase: classes of edge rows
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -626,11 +379,20 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
     } else if (KIND == TF_KIND_ELECTRODE) {
         if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+    } else if (KIND == TF_KIND_MULTIPHASE) {
+        for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
+        // the class ring's pad slots (no wave above the first / below the
+        // last) are read as class codes, i.e. table indices: keep them 0
+        for (int i = threadIdx.x; i < Sh::NR * 2 * 2 * kRowPairs; i += NW * 64) {
+            const int pl = i / (4 * kRowPairs), r = i % (4 * kRowPairs);
+            const int slot = r < 2 * kRowPairs ? 0 : NW + 1;
+            sKE[pl * Sh::KEB + slot * 2 * kRowPairs + r % (2 * kRowPairs)] = 0;
+        }
     }
     const float ky = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 2] : 0.0f;
     const float kz = KIND == TF_KIND_ANISOTROPIC ? hm[TF_ANISO_TABLE - 1] : 0.0f;
 
-    // ---- global memory (see sweep_tile for the descriptor conventions) ----
+    // ---- global memory (descriptor conventions: see the sweep section head) ----
     // loads: one 8-byte field load (the lane's pair) and one 2-byte code
     // load per row and plane, straight into the register ring
     constexpr int kOff = 0x7fffff00;
@@ -759,10 +521,17 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         // read by the first row of the wave below in every half-sweep of
         // that plane (LDS ring slot = register ring slot)
         auto publish_faces = [&](auto slot_c) {
+            constexpr int sl = decltype(slot_c)::value;
             if constexpr (kFaces) {
-                constexpr int sl = decltype(slot_c)::value;
                 reinterpret_cast<u32x2_t*>(sGE + sl * Sh::GEB)[(wave + 1) * kRowPairs + lane] =
                     G[sl][RPW - 1][1];
+            } else if constexpr (kClasses) {
+                // classes of the first (top) and last (bottom) row of a freshly
+                // loaded plane, read by the neighbouring waves in every
+                // half-sweep of that plane
+                uint16_t* ke = sKE + sl * Sh::KEB + (wave + 1) * 2 * kRowPairs + lane;
+                ke[0] = (uint16_t)K[sl][0];
+                ke[kRowPairs] = (uint16_t)K[sl][RPW - 1];
             }
         };
 
@@ -777,6 +546,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             u32x2_t gy0{0u, 0u};                    // y faces of the wave above's last row
             if constexpr (kFaces)
                 gy0 = reinterpret_cast<const u32x2_t*>(sGE + sl * Sh::GEB)[wave * kRowPairs + lane];
+            uint32_t ka = 0, kb = 0;                // classes: last row above, first row below
+            if constexpr (kClasses) {
+                const uint16_t* ke = sKE + sl * Sh::KEB + lane;
+                ka = ke[wave * 2 * kRowPairs + kRowPairs];
+                kb = ke[(wave + 2) * 2 * kRowPairs];
+            }
 #pragma unroll
             for (int j = 0; j < RPW; ++j) {
                 const int a = act(u, j);
@@ -817,6 +592,23 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     s = s - c;
                     s = s * w;
                     out = c + s;
+                } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
+                    // face conductances from the classes of the voxel and of
+                    // its six neighbours (MultiPhaseSolver, taufactor/taufactor.py:598-613)
+                    auto cls = [a](uint32_t k) { return a ? (k >> 8) & 0xffu : k & 0xffu; };
+                    const uint32_t kc = K[sl][j];
+                    const float* row = sT + cls(kc) * ncls;
+                    const uint32_t kyp = j < RPW - 1 ? K[sl][j + 1] : kb;
+                    const uint32_t kym = j > 0 ? K[sl][j - 1] : ka;
+                    const uint32_t czp =
+                        a ? (uint32_t)__builtin_amdgcn_update_dpp(0, (int)kc, 0x130, 0xf, 0xf, false) & 0xffu
+                          : (kc >> 8) & 0xffu;
+                    const uint32_t czm =
+                        a ? kc & 0xffu
+                          : ((uint32_t)__builtin_amdgcn_update_dpp(0, (int)kc, 0x138, 0xf, 0xf, false) >> 8) & 0xffu;
+                    const Nbr n{c, xp, xm, yp, ym, zp, zm};
+                    out = update_multiphase(n, row[cls(K[slp][j])], row[cls(K[slm][j])], row[cls(kyp)],
+                                            row[cls(kym)], row[czp], row[czm], p == 1, p == v.X, w);
                 } else if constexpr (KIND == TF_KIND_FIELD) {
                     // faces of the voxel: its own gx, gy, gz and those of its
                     // x-1 (previous plane), y-1 (row above) and z-1 (other
@@ -909,31 +701,23 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
             ? reinterpret_cast<const uint8_t*>(static_cast<const float*>(a.codes) + off)
             : static_cast<const uint8_t*>(a.codes) + (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
-    if constexpr (reg_kind(KIND))
-        rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
-                                      nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx,
-                                      nar, a.fsec);
-    else
-        sweep_tile<H, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.ncls, a.v, a.w, a.colour0,
-                               a.TX, tz, ty, tx);
+    rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
+                                  nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx, nar,
+                                  a.fsec, a.ncls);
 }
 
 // Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows
 // and z voxels per tile, steps per unrolled block, LDS bytes.
 template <int H, int KIND, int NW, int RPW>
 struct SweepGeom {
-    static constexpr bool reg = reg_kind(KIND);
-    static constexpr int TY = reg ? RegShape<H, NW, RPW>::TY : SweepShape<H, NW, RPW>::TY;
-    static constexpr int TZ = reg ? RegShape<H, NW, RPW>::TZ : SweepShape<H, NW, RPW>::TZ;
-    static constexpr int NP = reg ? RegShape<H, NW, RPW>::NR : SweepShape<H, NW, RPW>::NP;
+    static constexpr int TY = RegShape<H, NW, RPW>::TY;
+    static constexpr int TZ = RegShape<H, NW, RPW>::TZ;
+    static constexpr int NP = RegShape<H, NW, RPW>::NR;
 };
 
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
-    if (reg_kind(kind)) return RegShape<H, NW, RPW>::lds_bytes(kind);
-    // LDS ring (MultiPhase): field and class planes, then the face table
-    using Sh = SweepShape<H, NW, RPW>;
-    return kLdsPad + (size_t)Sh::NP * Sh::PS * (sizeof(float) + 1) + (size_t)ncls * ncls * sizeof(float);
+    return RegShape<H, NW, RPW>::lds_bytes(kind, ncls);
 }
 
 // Workgroups of one sweep kernel resident on the current device at once
@@ -989,7 +773,7 @@ static inline int balanced_tx(int X, long long yz_tiles, int H, int NP, int slot
 template <int H, int KIND, int NW, int RPW>
 static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
     using Sh = SweepGeom<H, KIND, NW, RPW>;
-    if constexpr (Sh::TY < (Sh::reg ? 2 : 1)) {
+    if constexpr (Sh::TY < 2) {
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
     } else {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
@@ -1007,7 +791,7 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
         a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
         a.nTyN = 0;
-        if constexpr (Sh::reg) {
+        {
             // wide tiles while a full one fits, then a narrow strip if the
             // rest fits one (RegShape::TZN), else one more wide tile
             const int tzn = RegShape<H, NW, RPW>::TZN;
@@ -1021,10 +805,10 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
             static int slots_cache[kMaxDevices] = {};
             const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
                                                   lds, slots_cache);
-            a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, Sh::reg);
+            a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, true);
         }
         // the register-blocked kernel needs an even x tile (rsweep_tile)
-        if (Sh::reg) a.TX = (a.TX + 1) & ~1;
+        a.TX = (a.TX + 1) & ~1;
         a.nTx = (a.v.X + a.TX - 1) / a.TX;
         const long long wide_tiles = (long long)a.nTz * a.nTy * a.nTx * B;
         const long long total = yz * a.nTx * B;
@@ -1049,7 +833,7 @@ static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) 
     if constexpr (H <= 4) {
         if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
     }
-    if constexpr (KIND == TF_KIND_SOLVER || KIND == TF_KIND_FIELD) {
+    if constexpr (KIND == TF_KIND_SOLVER || KIND == TF_KIND_FIELD || KIND == TF_KIND_MULTIPHASE) {
         if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
     }
     if constexpr (KIND == TF_KIND_SOLVER && H <= 4) {
