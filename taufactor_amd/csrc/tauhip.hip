@@ -398,8 +398,6 @@ extern "C" int tf_sweep_half_inplace(float* field, const uint8_t* codes, const f
     return 0;
 }
 
-// owned voxels of z per tile of the register-blocked kernel (RegShape::TZ)
-static int reg_tile_tz(int H) { return ((kRowLen - (H + (H & 1)) - H) / 2) * 2; }
 
 // Default workgroup shape.  32-row tiles (16 waves x 2) halve the y-halo
 // overhead of 16-row tiles where their LDS ring fits (one-byte codes,
@@ -890,9 +888,11 @@ extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_til
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS) return fail(-1, "n_half=%d out of range", n_half);
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
+    // the tiles k_sweep_batch decodes: wide tiles, then a narrow strip
     const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
-    const long long tz = reg_tile_tz(n_half);
-    long long n = ((Z + tz - 1) / tz) * ((Y + ty - 1) / ty) * ((X + t.tx - 1) / t.tx);
+    const ZTiling zt = z_tiling(Z, reg_tile_tz(n_half), reg_tile_tzn(n_half));
+    const long long nty = (Y + ty - 1) / ty;
+    long long n = ((long long)zt.wide * nty + (zt.narrow ? (nty + 1) / 2 : 0)) * ((X + t.tx - 1) / t.tx);
     if (n >= (1LL << 28)) return fail(-1, "too many tiles");
     *count = (int)n;
     return 0;

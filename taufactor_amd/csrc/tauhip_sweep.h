@@ -276,13 +276,28 @@ __host__ __device__ constexpr bool own_code_kind(int kind) {
 }
 
 
+// z tiling of the register-blocked sweep: wide tiles of TZ while a full one
+// fits, then one narrow strip (z0 = 1 + wide * TZ) if the rest fits TZN,
+// else one more wide tile
+struct ZTiling {
+    int wide;     // wide tiles
+    bool narrow;  // a narrow strip follows
+};
+__host__ __device__ inline ZTiling z_tiling(int Z, int TZ, int TZN) {
+    const int w = Z / TZ, rest = Z - w * TZ;
+    return {w + (rest > TZN ? 1 : 0), rest > 0 && rest <= TZN};
+}
+// owned z of a wide and of a narrow tile at depth H (RegShape::TZ, ::TZN)
+__host__ __device__ constexpr int reg_tile_tz(int H) { return ((128 - (H + (H & 1)) - H) / 2) * 2; }
+__host__ __device__ constexpr int reg_tile_tzn(int H) { return ((64 - (H + (H & 1)) - H) / 2) * 2; }
+
 template <int H, int NW, int RPW>
 struct RegShape {
     static constexpr int EY = NW * RPW;
     static constexpr int TY = EY - 2 * H;
     static constexpr int ML = H + (H & 1);                    // z0 - ew: even, so ew is odd
-    static constexpr int TZ = ((kRowLen - ML - H) / 2) * 2;   // owned voxels of z (even)
-    static constexpr int TZN = ((kRowPairs - ML - H) / 2) * 2;  // the same for a narrow tile
+    static constexpr int TZ = reg_tile_tz(H);    // owned voxels of z (even): 128 - ML - H
+    static constexpr int TZN = reg_tile_tzn(H);  // the same for a narrow tile: 64 - ML - H
     // planes in registers (even): H + 2 in use and D in flight; D = 1 at
     // H = 5 keeps a 16-wave workgroup within 128 VGPRs per lane
     static constexpr int NR = H == 5 ? 8 : ((H + 5) / 2) * 2;
@@ -792,13 +807,10 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
         a.nTyN = 0;
         {
-            // wide tiles while a full one fits, then a narrow strip if the
-            // rest fits one (RegShape::TZN), else one more wide tile
-            const int tzn = RegShape<H, NW, RPW>::TZN;
-            const int wide = a.v.Z / Sh::TZ, rest = a.v.Z - wide * Sh::TZ;
-            a.nTz = wide + (rest > tzn ? 1 : 0);
-            if (rest > 0 && rest <= tzn) a.nTyN = (a.nTy + 1) / 2;
-            a.z0N = 1 + wide * Sh::TZ;
+            const ZTiling zt = z_tiling(a.v.Z, Sh::TZ, RegShape<H, NW, RPW>::TZN);
+            a.nTz = zt.wide;
+            if (zt.narrow) a.nTyN = (a.nTy + 1) / 2;
+            a.z0N = 1 + (a.v.Z / Sh::TZ) * Sh::TZ;
         }
         const long long yz = (long long)a.nTz * a.nTy + a.nTyN;
         if (a.TX <= 0) {
@@ -890,17 +902,31 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
     if (!a.state[vi].active) return;
     const tf_volume vol = a.vols[vi];
     const Vol v = make_vol(vol.X, vol.Y, vol.Z);
-    const int nTz = (vol.Z + Sh::TZ - 1) / Sh::TZ;
+    // this volume's tiles, numbered as in k_sweep: wide tiles (tz fastest,
+    // then ty, then tx), then the narrow strip (ty, then tx)
+    const ZTiling zt = z_tiling(vol.Z, Sh::TZ, Sh::TZN);
     const int nTy = (vol.Y + Sh::TY - 1) / Sh::TY;
+    const int nTx = (vol.X + a.TX - 1) / a.TX;
     int t = logical - vol.tile_first;
-    const int tz = t % nTz; t /= nTz;
-    const int ty = t % nTy;
-    const int tx = t / nTy;
+    const int n_wide = zt.wide * nTy * nTx;
+    const bool nar = t >= n_wide;
+    int z0, ty, tx;
+    if (!nar) {  // (zt.wide >= 1 here)
+        z0 = 1 + (t % zt.wide) * Sh::TZ;
+        t /= zt.wide;
+        ty = t % nTy;
+        tx = t / nTy;
+    } else {
+        t -= n_wide;
+        const int nTyN = (nTy + 1) / 2;
+        z0 = 1 + (vol.Z / Sh::TZ) * Sh::TZ;
+        ty = t % nTyN;
+        tx = t / nTyN;
+    }
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
     rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
-                                            vol.omega, a.colour0, a.TX, 1 + tz * Sh::TZ, ty, tx,
-                                            false);
+                                            vol.omega, a.colour0, a.TX, z0, ty, tx, nar);
 }
 
 template <int H, int NW, int RPW>
