@@ -9,7 +9,7 @@ and keeps the one with the lowest time per reference iteration:
   each candidate on scratch copies of the current field, so the solver's
   state is untouched;
 * BatchSolver: a few passes of each candidate over the whole packed batch
-  (H must be even and divide 100 there), after which the batch is re-packed.
+  (H must divide 100 there), after which the batch is re-packed.
 
 Results are cached per (kind, batch, shape) for the life of the process.
 Volumes below MIN_VOXELS are not tuned: the defaults (H = 4, library tile)
@@ -30,7 +30,7 @@ TIMED_LAUNCHES = 3
 # length for the shape and volume.
 SOLVER_SPACE = ((5, 4, 6, 3, 2), ((16, 2), (12, 2), (8, 2)), (0, 128, 64))
 FIELD_SPACE = ((4, 3, 2), ((8, 2), (12, 2), (16, 2)), (0, 128, 64))
-BATCH_SPACE = ((4, 2), ((8, 2), (4, 4), (12, 2), (16, 2)), (64, 32))
+BATCH_SPACE = ((5, 4, 2), ((8, 2), (4, 4), (12, 2), (16, 2)), (64, 32))
 TIE = 0.02
 
 _cache: dict = {}

@@ -59,7 +59,7 @@ class BatchSolver:
         self._lib = _lib.load()
         self.D_0 = D_0
         # launch configuration: None lets the autotuner (large batches) or the
-        # defaults (H = 4, 16-row tiles) decide; H must be even and divide 100
+        # defaults (H = 4, 16-row tiles) decide; H must divide 100
         self.half_sweeps_per_pass = None
         self.tile = None
         self.autotune = True
@@ -186,8 +186,10 @@ class BatchSolver:
         """100 iterations (passes + check) starting from buffer src_b; returns the
         buffer holding the result."""
         H = self._plan["H"]
-        for _ in range(CHECK_EVERY // H):
-            self._pass(src_b, 0, H)  # every pass starts on colour 0 (H is even)
+        for k in range(CHECK_EVERY // H):
+            # a period starts on colour 0 (100 is even); pass k after k*H
+            # half-sweeps
+            self._pass(src_b, (k * H) & 1, H)
             src_b ^= 1
         self._check(src_b, iter_limit, conv_crit)
         return src_b
@@ -212,8 +214,8 @@ class BatchSolver:
         """Iterate every volume until its own convergence (or iter_limit, a
         multiple of 100).  Returns tau (float32 [V])."""
         H, tile = self._launch_config()
-        if H % 2 or CHECK_EVERY % H or H > _lib.MAX_FUSED_HALF_SWEEPS:
-            raise ValueError("half_sweeps_per_pass must be even and divide 100")
+        if CHECK_EVERY % H or H > _lib.MAX_FUSED_HALF_SWEEPS:
+            raise ValueError("half_sweeps_per_pass must divide 100 (1, 2, 4 or 5)")
         if iter_limit % CHECK_EVERY:
             raise ValueError("BatchSolver needs iter_limit to be a multiple of 100")
         with torch.no_grad(), torch.cuda.device(self.device):
