@@ -191,7 +191,11 @@ int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int kind, in
  *            (TF_KIND_MULTIPHASE), ignored for TF_KIND_SOLVER
  *   omega    the over-relaxation factor rounded to float32
  *   n_half   1..TF_MAX_FUSED_HALF_SWEEPS
- *   tile     NULL for the default tile shape                              */
+ *   tile     NULL for the default tile shape
+ * Sizes: a padded volume holds fewer than 2^31 voxels (every entry point);
+ * the sweep addresses each tile from its first plane, so a volume may exceed
+ * 2 GiB per field as long as (tx + 2 n_half) planes stay below 2 GiB (the x
+ * tile is capped to that).                                                  */
 #define TF_MAX_FUSED_HALF_SWEEPS 6
 int tf_sweep(const float* src, float* dst, const void* codes,
              const float* hm_lut, int n_classes, int kind,

@@ -432,10 +432,11 @@ static int sweep_prepare(const float* hm_lut, int n_classes, int kind, int B, in
         if (tile->tx > 0) t->tx = tile->tx;
     }
     if (t->tx < 0 || t->tx > 65536) return fail(-1, "invalid tile tx=%d", t->tx);
-    if ((long long)(X + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64 ||
-        sweep_code_volume(X, Y, Z) >= (1LL << 31))
-        return fail(-1, "padded volume of %lld bytes: the sweep addresses at most 2 GiB per volume",
-                    (long long)(X + 2) * (Y + 2) * (Z + 2) * 4);
+    if (sweep_code_volume(X, Y, Z) >= (1LL << 31))
+        return fail(-1, "volume too large: %lld bytes of sweep codes", sweep_code_volume(X, Y, Z));
+    // a tile's loads address its planes with 32-bit offsets
+    if ((2LL * n_half + 2) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64)
+        return fail(-1, "a %dx%d plane is too large for the sweep", Y, Z);
     return 0;
 }
 
@@ -889,6 +890,8 @@ extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_til
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
     // the tiles k_sweep_batch decodes: wide tiles, then a narrow strip
+    if ((long long)(t.tx + 2 * n_half) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64)
+        return fail(-1, "x tile %d of %dx%d planes exceeds the sweep's 2 GiB tile span", t.tx, Y, Z);
     const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
     const ZTiling zt = z_tiling(Z, reg_tile_tz(n_half), reg_tile_tzn(n_half));
     const long long nty = (Y + ty - 1) / ty;

@@ -412,17 +412,23 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     // load per row and plane, straight into the register ring
     constexpr int kOff = 0x7fffff00;
     const int plane_f = v.sX * 4;
-    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S) - kSlackBytes,
-                                (uint32_t)(v.sB * 4) + 2u * kSlackBytes);
+    // the descriptors start at the tile's first streamed plane (pbase), so
+    // offsets stay within the tile's planes (< 2 GiB, checked by the
+    // launch) whatever the size of the volume
+    const int pbase = max(x0 - H, 0);
+    const uint32_t span_f = (uint32_t)(lx1 - pbase) * (uint32_t)plane_f;
+    const rsrc_t rF = make_rsrc(reinterpret_cast<const char*>(S + (size_t)pbase * v.sX) - kSlackBytes,
+                                span_f + 2u * kSlackBytes);
     const int row_c = sweep_code_row(v.Z);
     const int plane_c = (v.Y + 2) * row_c;
-    const rsrc_t rC = make_rsrc(Cg, (uint32_t)((v.X + 2) * plane_c));
+    const rsrc_t rC = make_rsrc(Cg + (size_t)pbase * plane_c, (uint32_t)(lx1 - pbase) * (uint32_t)plane_c);
     rsrc_t rG[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        rG[k] = make_rsrc(reinterpret_cast<const char*>(reinterpret_cast<const float*>(Cg) + k * fsec) -
+        rG[k] = make_rsrc(reinterpret_cast<const char*>(reinterpret_cast<const float*>(Cg) + k * fsec +
+                                                        (size_t)pbase * v.sX) -
                               kSlackBytes,
-                          (uint32_t)(v.sB * 4) + 2u * kSlackBytes);
+                          span_f + 2u * kSlackBytes);
     int vf[RPW], vc[RPW];
     bool own_row[2][RPW];   // wave-uniform, per half
     int ys[2][RPW];
@@ -459,8 +465,8 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             for (int k = 0; k < 3; ++k) G[i][j][k] = u32x2_t{0u, 0u};
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
-        const bool pok = p >= 0 && p < lx1;
-        const int sf = pok ? p * plane_f : 0, sc = pok ? p * plane_c : 0;
+        const bool pok = p >= pbase && p < lx1;
+        const int sf = pok ? (p - pbase) * plane_f : 0, sc = pok ? (p - pbase) * plane_c : 0;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
             F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, pok ? vf[j] : kOff, sf, 0);
@@ -485,7 +491,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         for (int j = 0; j < RPW; ++j) {
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                const rsrc_t r = make_rsrc(D + qs * v.sX + ys[k][j] * v.sY,
+                const rsrc_t r = make_rsrc(D + (size_t)qs * v.sX + (size_t)ys[k][j] * v.sY,
                                            (qok && own_row[k][j]) ? (uint32_t)(4 * z1) : 0u);
                 __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, k ? zs1 : zs0, 0, 0);
             }
@@ -819,8 +825,12 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
                                                   lds, slots_cache);
             a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, true);
         }
-        // the register-blocked kernel needs an even x tile (rsweep_tile)
+        // the register-blocked kernel needs an even x tile (rsweep_tile),
+        // and a tile's planes must span less than 2 GiB (32-bit offsets)
         a.TX = (a.TX + 1) & ~1;
+        const long long tx_max = (((1LL << 31) - 64) / ((long long)a.v.sX * 4) - 2 * H) & ~1LL;
+        if (tx_max < 2) return fail(-1, "a plane of %d voxels is too large for the sweep", a.v.sX);
+        if (a.TX > tx_max) a.TX = (int)tx_max;
         a.nTx = (a.v.X + a.TX - 1) / a.TX;
         const long long wide_tiles = (long long)a.nTz * a.nTy * a.nTx * B;
         const long long total = yz * a.nTx * B;

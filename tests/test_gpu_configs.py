@@ -124,3 +124,22 @@ def test_balanced_x_tile_at_512():
     st = lib.tf_sweep_launch_config(_lib.KIND_SOLVER, 0, 1, 512, 512, 512, 3, _lib.Tile(16, 2, 104),
                                     ctypes.byref(out), ctypes.byref(n))
     assert st == 0 and out.tx == 104 and n.value == 450
+
+
+def test_volume_above_2gib_per_field():
+    """A 560 x 1024 x 1024 volume (2.36 GB per padded float32 field): the
+    sweep's buffer descriptors start at each tile's first plane, so offsets
+    stay 32-bit whatever the volume size.  Two iterations (one launch of two
+    fused half-sweeps) equal the oracle's, value for value."""
+    from oracle.restate import OracleSolver
+
+    rng = np.random.default_rng(11)
+    img = (rng.random((560, 1024, 1024), dtype=np.float32) < 0.6).astype(np.uint8)
+    s = tau.Solver(img)
+    s.half_sweeps_per_pass = 2
+    s.solve(iter_limit=2, verbose=False)
+    got = s.field.cpu().numpy()
+    del s
+    torch.cuda.empty_cache()
+    o = OracleSolver(img).run(2)
+    assert np.array_equal(got, o.field.numpy())
