@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 6
+#define TF_ABI_VERSION 7
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -153,9 +153,17 @@ int tf_setup_electrode(const uint8_t* cls, const float* xinit, uint8_t* codes,
  * the opposite interior layer (corners both wraps).  Replaces
  * apply_boundary_conditions of PeriodicSolver / PeriodicMultiPhaseSolver
  * (taufactor/taufactor.py:501-505, 652-656), called before every
- * iteration: a periodic solve alternates tf_wrap_ghosts on the source buffer
- * and a one-half-sweep tf_sweep.                                          */
+ * iteration.  With odd Y or Z a periodic solve alternates tf_wrap_ghosts on
+ * the source buffer and a one-half-sweep tf_sweep; with Y and Z even it
+ * sweeps with TF_SWEEP_PERIODIC_YZ instead (no ghost is read).            */
 int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream);
+
+/* The same copy from the interior of src into the ghosts of dst (src ==
+ * dst is tf_wrap_ghosts).  After a TF_SWEEP_PERIODIC_YZ sweep whose last
+ * launch is one half-sweep src -> dst, it leaves in dst the ghosts the
+ * reference holds after that iteration (the wrap made before it).  No
+ * reference counterpart as an entry point.                                */
+int tf_wrap_ghosts_from(const float* src, float* dst, int B, int X, int Y, int Z, void* stream);
 
 /* Size in bytes of the sweep-code layout of one volume: (X+2)*(Y+2) rows of
  * tf_sweep_code_row(Z) = roundup(Z+5, 4) bytes; byte z+3 of row (x, y)
@@ -197,6 +205,16 @@ int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int kind, in
  * 2 GiB per field as long as (tx + 2 n_half) planes stay below 2 GiB (the x
  * tile is capped to that).                                                  */
 #define TF_MAX_FUSED_HALF_SWEEPS 6
+/* Flag OR-ed into the kind of tf_sweep / tf_sweep_launch_config: periodic y
+ * and z.  Every y/z neighbour across the boundary is read from the opposite
+ * interior layer (the value tf_wrap_ghosts would copy into the ghost before
+ * each iteration, taufactor/taufactor.py:501-505, 652-656), so n_half fused
+ * half-sweeps equal n_half periodic reference iterations; the ghosts of src
+ * are not read and those of dst not written.  TF_KIND_SOLVER,
+ * TF_KIND_MULTIPHASE and TF_KIND_ELECTRODE with codes set up with
+ * TF_SETUP_PERIODIC_YZ; Y and Z must be even (the red/black colouring then
+ * repeats across the boundary).                                            */
+#define TF_SWEEP_PERIODIC_YZ 0x100
 int tf_sweep(const float* src, float* dst, const void* codes,
              const float* hm_lut, int n_classes, int kind,
              int B, int X, int Y, int Z, float omega, int first_colour,

@@ -67,7 +67,8 @@ def _time(launch, n):
 def tune_solver(s, fixed_h=None, fixed_tile=None):
     """Best (H, tile) for a plain solver `s` (SORSolver subclass)."""
     B, X, Y, Z = s.batch_size, s.Nx, s.Ny, s.Nz
-    key = ("plain", s.kind, B, X, Y, Z, fixed_h, None if fixed_tile is None else tuple(fixed_tile))
+    kind = s.kind | s._sweep_flags
+    key = ("plain", kind, B, X, Y, Z, fixed_h, None if fixed_tile is None else tuple(fixed_tile))
     if key in _cache:
         return _cache[key]
     lib = s._lib
@@ -90,7 +91,7 @@ def tune_solver(s, fixed_h=None, fixed_tile=None):
             i = state["i"]
             state["i"] += 1
             return lib.tf_sweep(_lib.ptr(bufs[i & 1]), _lib.ptr(bufs[1 - (i & 1)]), codes, hm,
-                                s._n_classes, s.kind, B, X, Y, Z, s._w32, 0, H, t, stream)
+                                s._n_classes, kind, B, X, Y, Z, s._w32, 0, H, t, stream)
 
         ms = _time(launch, TIMED_LAUNCHES)
         if ms is not None:

@@ -256,7 +256,7 @@ extern "C" int tf_setup_solver(const uint8_t* mask, const float* xinit, uint8_t*
 // y/z ghosts of every x plane := the opposite interior layer (ghost corners
 // take both wraps), as SORSolver.apply_boundary_conditions of the periodic
 // solvers does before each iteration (taufactor/taufactor.py:501-505, 652-656)
-__global__ void k_wrap_ghosts(float* __restrict__ F, Vol v, int B) {
+__global__ void k_wrap_ghosts(const float* S, float* D, Vol v, int B) {
     // one thread per ghost position of one (b, x) plane: the border of the
     // (Y+2) x (Z+2) plane, 2(Y+2) + 2Z positions
     const int per_plane = 2 * (v.Y + 2) + 2 * v.Z;
@@ -274,20 +274,27 @@ __global__ void k_wrap_ghosts(float* __restrict__ F, Vol v, int B) {
     }
     const int ys = yp == 0 ? v.Y : (yp == v.Y + 1 ? 1 : yp);
     const int zs = zp == 0 ? v.Z : (zp == v.Z + 1 ? 1 : zp);
-    float* P = F + (size_t)bx * v.sX;
-    P[yp * v.sY + zp] = P[ys * v.sY + zs];
+    const size_t plane = (size_t)bx * v.sX;
+    D[plane + yp * v.sY + zp] = S[plane + ys * v.sY + zs];
 }
 
-extern "C" int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream) {
+// ghosts of dst from the interior of src (src == dst: in place; the ghosts
+// are neither read nor written by the copies of other ghosts)
+extern "C" int tf_wrap_ghosts_from(const float* src, float* dst, int B, int X, int Y, int Z,
+                                   void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
-    if (!field) return fail(-1, "null pointer argument");
+    if (!src || !dst) return fail(-1, "null pointer argument");
     if ((long long)B * (X + 2) > 65535) return fail(-1, "grid too large for ghost wrap");
     Vol v = make_vol(X, Y, Z);
     const int per_plane = 2 * (Y + 2) + 2 * Z;
     hipLaunchKernelGGL(k_wrap_ghosts, dim3((per_plane + 255) / 256, B * (X + 2)), dim3(256), 0,
-                       (hipStream_t)stream, field, v, B);
+                       (hipStream_t)stream, src, dst, v, B);
     TF_LAUNCH_CHECK("k_wrap_ghosts");
     return 0;
+}
+
+extern "C" int tf_wrap_ghosts(float* field, int B, int X, int Y, int Z, void* stream) {
+    return tf_wrap_ghosts_from(field, field, B, X, Y, Z, stream);
 }
 
 extern "C" int tf_setup_multiphase(const uint8_t* cls, const uint8_t* cls_conductive,
@@ -410,6 +417,21 @@ static void default_tile(int H, int kind, int Y, tf_tile* t) {
     t->tx = 0;  // chosen at launch from the resident workgroups (balanced_tx)
 }
 
+// The kind argument of tf_sweep / tf_sweep_launch_config: the kind in the
+// low byte, TF_SWEEP_PERIODIC_YZ above it.
+static int split_sweep_kind(int& kind, int& periodic, int Y, int Z) {
+    if (kind & ~(0xff | TF_SWEEP_PERIODIC_YZ)) return fail(-1, "unknown sweep flags in kind 0x%x", kind);
+    periodic = (kind & TF_SWEEP_PERIODIC_YZ) ? 1 : 0;
+    kind &= 0xff;
+    if (periodic) {
+        if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_ELECTRODE)
+            return fail(-1, "TF_SWEEP_PERIODIC_YZ applies to the solver, multiphase and electrode kinds");
+        if ((Y | Z) & 1)
+            return fail(-1, "TF_SWEEP_PERIODIC_YZ needs even Y and Z (got %d, %d)", Y, Z);
+    }
+    return 0;
+}
+
 // Checks shared by tf_sweep and tf_sweep_launch_config; fills the shape.
 static int sweep_prepare(const float* hm_lut, int n_classes, int kind, int B, int X, int Y, int Z,
                          int n_half, const tf_tile* tile, tf_tile* t) {
@@ -457,6 +479,8 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
                         int n_classes, int kind, int B, int X, int Y, int Z, float omega,
                         int first_colour, int n_half, const tf_tile* tile, void* stream) {
     tf_tile t;
+    int periodic = 0;
+    if (int rc = split_sweep_kind(kind, periodic, Y, Z)) return rc;
     if (int rc = sweep_prepare(hm_lut, n_classes, kind, B, X, Y, Z, n_half, tile, &t)) return rc;
     if (!src || !dst || !codes) return fail(-1, "null pointer argument");
     if (src == dst) return fail(-1, "tf_sweep needs distinct src and dst buffers");
@@ -468,6 +492,7 @@ extern "C" int tf_sweep(const float* src, float* dst, const void* codes, const f
     a.colour0 = first_colour;
     a.TX = t.tx;
     a.fsec = (long long)B * a.v.sB;
+    a.periodic = periodic;
     return sweep_dispatch(kind, n_half, a, B, t, (hipStream_t)stream);
 }
 
@@ -478,6 +503,8 @@ extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int
     // the tables are not read here: any non-null pointer passes the checks
     static const float dummy = 0.0f;
     tf_tile t;
+    int periodic = 0;
+    if (int rc = split_sweep_kind(kind, periodic, Y, Z)) return rc;
     if (int rc = sweep_prepare(&dummy, n_classes, kind, B, X, Y, Z, n_half, tile, &t)) return rc;
     SweepArgs a = {};
     a.v = make_vol(X, Y, Z);

@@ -217,6 +217,7 @@ struct SweepArgs {
     int nWide, nTyN, z0N;
     long long fsec;    // field kind: floats between the gx, gy, gz face sections
     int resolve_only;  // set TX and the tile counts, launch nothing
+    int periodic;      // y/z neighbours wrap round (Y and Z even)
 };
 
 // compile-time loop: f(std::integral_constant<int, I>{}) for I in [B, E)
@@ -355,7 +356,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
                                             int colour0, int TX, int z0, int ty, int tx, bool nar,
-                                            long long fsec = 0, int ncls = 0) {
+                                            long long fsec = 0, int ncls = 0, bool wrap = false) {
     using Sh = RegShape<H, NW, RPW>;
     // field kind: Cg holds three sections of face conductances in the padded
     // field layout, fsec floats apart: gx, gy, gz, each stored at the voxel
@@ -432,12 +433,24 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     int vf[RPW], vc[RPW];
     bool own_row[2][RPW];   // wave-uniform, per half
     int ys[2][RPW];
+    // periodic y/z (wrap): every row and z pair of the tile, ghosts and halo
+    // beyond them included, is loaded from the interior voxels it repeats
+    // ((p - 1) mod N + 1), codes included, so it is updated exactly as they
+    // are; with Y and Z even the red/black colouring repeats as well.  The
+    // lane's pair starts at an odd z, and Z is even, so the pair never
+    // straddles the wrap.
+    auto wrap_at = [](int p, int n) {
+        int r = (p - 1) % n;
+        return (r < 0 ? r + n : r) + 1;
+    };
+    const int zr = wrap ? wrap_at(zl, v.Z) : zl;
 #pragma unroll
     for (int j = 0; j < RPW; ++j) {
         const int y = ey0 + wave * RPW + j;
-        const bool rok = y >= 0 && y <= v.Y + 1;
-        vf[j] = rok ? kSlackBytes + (y * v.sY + zl) * 4 : kOff;
-        vc[j] = rok ? y * row_c + zl + 3 : kOff;
+        const bool rok = wrap || (y >= 0 && y <= v.Y + 1);
+        const int yr = wrap ? wrap_at(y, v.Y) : y;
+        vf[j] = rok ? kSlackBytes + (yr * v.sY + zr) * 4 : kOff;
+        vc[j] = rok ? yr * row_c + zr + 3 : kOff;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int yk = y0h[k] - H + wave * RPW + j;
@@ -724,7 +737,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
     rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
                                   nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx, nar,
-                                  a.fsec, a.ncls);
+                                  a.fsec, a.ncls, a.periodic != 0);
 }
 
 // Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows

@@ -55,6 +55,7 @@ class SORSolver(ABC):
 
     kind: int = _lib.KIND_SOLVER
     _setup_flags: int = 0
+    _sweep_flags: int = 0  # OR-ed into the kind of tf_sweep (SWEEP_PERIODIC_YZ)
 
     def __init__(self, img: np.ndarray, omega: float | None = None, precision=None, device="cuda"):
         self.cpu_img = self._expand_to_4d(img)
@@ -199,7 +200,7 @@ class SORSolver(ABC):
         H = max(1, min(H, _lib.MAX_FUSED_HALF_SWEEPS))
         out, n = _lib.Tile(), ctypes.c_int(0)
         req = _lib.Tile(*tile) if tile is not None else None
-        st = self._lib.tf_sweep_launch_config(self.kind, self._n_classes, self.batch_size, self.Nx,
+        st = self._lib.tf_sweep_launch_config(self.kind | self._sweep_flags, self._n_classes, self.batch_size, self.Nx,
                                               self.Ny, self.Nz, H, req, ctypes.byref(out),
                                               ctypes.byref(n))
         _lib.check(st, "tf_sweep_launch_config")
@@ -233,8 +234,8 @@ class SORSolver(ABC):
             h = min(per, n_half)
             src = self._fields[self._cur]
             dst = self._fields[1 - self._cur]
-            st = lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes, self.kind,
-                              B, X, Y, Z, self._w32, it & 1, h,
+            st = lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes,
+                              self.kind | self._sweep_flags, B, X, Y, Z, self._w32, it & 1, h,
                               None if tile is None else tile, stream)
             _lib.check(st, "tf_sweep")
             self._cur = 1 - self._cur
@@ -845,13 +846,31 @@ class FieldSolver(ThroughTransportSolver):
 
 
 class _PeriodicYZ:
-    """Periodic y/z boundaries: the y/z ghosts are refreshed from the opposite
-    interior layer before every iteration (apply_boundary_conditions,
-    taufactor/taufactor.py:501-505, 652-656), so the solve alternates one
-    ghost wrap and one half-sweep."""
+    """Periodic y/z boundaries (apply_boundary_conditions before every
+    iteration, taufactor/taufactor.py:501-505, 652-656).
+
+    With Y and Z even the sweep reads every neighbour across the boundary
+    from the opposite interior layer (TF_SWEEP_PERIODIC_YZ), which is what
+    the refreshed ghost holds, so half-sweeps fuse as in the open solvers.
+    The last half-sweep before each convergence check runs on its own and
+    its source's interior is then copied into the ghosts of the result, so
+    the ghosts too end as the reference's: the copy made before the last
+    iteration.  With Y or Z odd the colouring does not repeat across the
+    boundary and the solve alternates one ghost wrap and one half-sweep."""
 
     _setup_flags = _lib.SETUP_PERIODIC_YZ
-    _tunable = False  # one half-sweep per ghost wrap: there is nothing to fuse
+
+    @property
+    def _fused_periodic(self) -> bool:
+        return self.Ny % 2 == 0 and self.Nz % 2 == 0
+
+    @property
+    def _tunable(self) -> bool:
+        return self._fused_periodic
+
+    @property
+    def _sweep_flags(self) -> int:
+        return _lib.SWEEP_PERIODIC_YZ if self._fused_periodic else 0
 
     def apply_boundary_conditions(self):
         st = self._lib.tf_wrap_ghosts(_lib.ptr(self.field), self.batch_size, self.Nx, self.Ny, self.Nz,
@@ -859,6 +878,20 @@ class _PeriodicYZ:
         _lib.check(st, "tf_wrap_ghosts")
 
     def _advance(self, n_half: int) -> None:
+        if self._fused_periodic:
+            it = self.iter
+            try:
+                if n_half > 1:
+                    SORSolver._advance(self, n_half - 1)
+                    self.iter = it + n_half - 1  # colour of the last iteration
+                src = self.field
+                SORSolver._advance(self, 1)
+            finally:
+                self.iter = it
+            st = self._lib.tf_wrap_ghosts_from(_lib.ptr(src), _lib.ptr(self.field), self.batch_size,
+                                               self.Nx, self.Ny, self.Nz, self._stream())
+            _lib.check(st, "tf_wrap_ghosts_from")
+            return
         ev = self._launch_events
         if ev is not None:
             t0 = torch.cuda.Event(enable_timing=True)

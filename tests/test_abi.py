@@ -43,6 +43,16 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert st < 0 and b"distinct" in lib.tf_last_error()
     st = lib.tf_sweep(1, 2, 3, None, 0, 0, 1, 4, 4, 4, 1.5, 0, 99, None, None)
     assert st < 0 and b"n_half" in lib.tf_last_error()
+    # the periodic sweep flag: even Y and Z, code kinds only, no other flag bits
+    per = _lib.SWEEP_PERIODIC_YZ
+    for kind, Y, Z in [(_lib.KIND_SOLVER, 7, 8), (_lib.KIND_MULTIPHASE, 8, 9),
+                       (_lib.KIND_FIELD, 8, 8), (_lib.KIND_ANISOTROPIC, 8, 8)]:
+        st = lib.tf_sweep(1, 2, 3, None, 0, kind | per, 1, 4, Y, Z, 1.5, 0, 2, None, None)
+        assert st < 0 and b"TF_SWEEP_PERIODIC_YZ" in lib.tf_last_error()
+    st = lib.tf_sweep(1, 2, 3, None, 0, 0x200, 1, 4, 4, 4, 1.5, 0, 2, None, None)
+    assert st < 0 and b"flags" in lib.tf_last_error()
+    st = lib.tf_wrap_ghosts_from(None, 1, 1, 4, 4, 4, None)
+    assert st < 0 and b"null pointer" in lib.tf_last_error()
 
 
 def test_spans_x_host_entry_point():

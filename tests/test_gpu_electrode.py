@@ -45,6 +45,18 @@ def test_periodic_iterates_and_labels():
     assert np.array_equal(s.field.cpu().numpy()[:, :, 1:-1, 1:-1], o.field.numpy()[:, :, 1:-1, 1:-1])
 
 
+@pytest.mark.parametrize("H", [1, 3, 6])
+def test_periodic_fused_iterates_bit_identical(H):
+    # Y and Z even: the sweep reads across the boundary and fuses; the whole
+    # field, ghosts included, equals the oracle's
+    img = _labels((11, 8, 10), 5 + H)
+    s = tau.PeriodicElectrodeSolver(img, conductive_label=5, reactive_label=1)
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=31, verbose=False)
+    o = OraclePeriodicElectrodeSolver(img, conductive_label=5, reactive_label=1).run(31)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
 def test_full_solve_matches_oracle():
     img = _labels((24, 16, 18), 9)
     s = tau.ElectrodeSolver(img)

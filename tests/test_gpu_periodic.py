@@ -1,8 +1,12 @@
-"""GPU: PeriodicSolver / PeriodicMultiPhaseSolver against the oracle
-(interiors value for value; the ghosts of the final field hold the copy made
-before the last iteration in both, but are compared only through the
-interior they feed), the reference's periodic tests
-(tests/test_taufactor.py:80-106, 226-247) and the stored notebook table."""
+"""GPU: PeriodicSolver / PeriodicMultiPhaseSolver against the oracle, the
+reference's periodic tests (tests/test_taufactor.py:80-106, 226-247) and the
+stored notebook table.
+
+With Y and Z even the sweep reads across the boundary and fuses half-sweeps;
+the whole field, ghosts included, then equals the oracle's (the ghosts hold
+the copy made before the last iteration).  With Y or Z odd the solve
+alternates a ghost wrap and one half-sweep, and the interiors are compared
+(the ghosts of the final field are a copy taken at another point)."""
 import numpy as np
 import pytest
 
@@ -73,3 +77,62 @@ def test_periodic_notebook_table(goldens, which):
     S.solve(conv_crit=1e-3, verbose=False)
     assert S.iter == row["iterations"]
     assert abs(float(S.tau[0]) - row["tau"]) <= 1.5e-6 * row["tau"]
+
+
+def _binary(shape, seed, p=0.6):
+    rng = np.random.default_rng(seed)
+    return (rng.random(shape) < p).astype(np.uint8)
+
+
+@pytest.mark.parametrize("shape", [(9, 10, 12), (17, 2, 4), (6, 4, 130), (13, 30, 2), (5, 6, 8)])
+@pytest.mark.parametrize("H", [1, 2, 3, 5, 6])
+def test_periodic_fused_bit_identical(shape, H):
+    """Y smaller than the halo (the halo wraps several times), Z = 2, and
+    z covered by more than one tile; two solve calls, so the second starts on
+    the other colour."""
+    img = _binary(shape, sum(shape) * 7 + H)
+    s = tau.PeriodicSolver(img)
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=5, verbose=False)
+    s.solve(iter_limit=5 + 2 * H + 3, verbose=False)
+    o = OraclePeriodicSolver(img).run(5 + 2 * H + 3)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
+@pytest.mark.parametrize("H,Z", [(4, 176), (4, 240), (5, 168), (6, 200)])
+@pytest.mark.parametrize("tile", [(16, 2, 4), (8, 2, 0), (12, 2, 6)])
+def test_periodic_fused_z_tiles(H, Z, tile):
+    """Wide tiles, the narrow last strip (Y = 34 gives an odd count of
+    16-row y tiles) and wraps at both ends of z."""
+    img = _binary((9, 34, Z), Z + H)
+    s = tau.PeriodicSolver(img)
+    s.half_sweeps_per_pass = H
+    s.tile = tile
+    s.solve(iter_limit=2 * H + 1, verbose=False)
+    o = OraclePeriodicSolver(img).run(2 * H + 1)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
+@pytest.mark.parametrize("H", [2, 4, 6])
+def test_periodic_multiphase_fused_bit_identical(H):
+    rng = np.random.default_rng(40 + H)
+    img = rng.integers(0, 3, size=(9, 10, 12))
+    Ds = {0: 0.0, 1: 1.0, 2: 0.2}
+    s = tau.PeriodicMultiPhaseSolver(img, dict(Ds))
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=37, verbose=False)
+    o = OraclePeriodicMultiPhaseSolver(img, dict(Ds)).run(37)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
+def test_periodic_fused_full_solve_and_autotune():
+    """A full solve through the autotuned fused sweep stops where the oracle
+    does with the same tau."""
+    img = _binary((40, 32, 36), 3)
+    s = tau.PeriodicSolver(img)
+    s.solve(verbose=False)
+    o = OraclePeriodicSolver(img).solve()
+    assert s.iter == o.iter
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+    np.testing.assert_allclose(s.tau, o.tau, rtol=2e-7)
+

@@ -10,7 +10,8 @@ s updates plane s-h; z neighbours across lanes come from the adjacent lane
 (the DPP wave shifts; 0 past the ends), the y neighbours of a wave's first and
 last row from the edge values the neighbouring waves published at step s-1,
 double-buffered by step parity; rows and z are updated unconditionally, planes
-only inside each half-sweep's range.  A wrong ring slot, parity, publication
+only inside each half-sweep's range.  With wrap (periodic y/z) every row
+and z position loads the interior voxel it repeats.  A wrong ring slot, parity, publication
 step or narrow-tile row offset changes the stored field, which must equal H
 reference iterations of the oracle (Solver kind, IEEE division in place of the
 exact reciprocal sequence, which gives the same quotient).
@@ -23,7 +24,11 @@ from oracle.restate import OracleSolver
 L = 64
 
 
-def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None):
+def _wrap_at(p, n):
+    return (p - 1) % n + 1
+
+
+def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None, wrap=False):
     X, Y, Z = (n - 2 for n in f0.shape)
     sX, sY = (Y + 2) * (Z + 2), Z + 2
     flat = np.concatenate([np.zeros(4, np.float32), f0.ravel(), np.zeros(4, np.float32)])
@@ -66,6 +71,8 @@ def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None):
                         for e in range(2):
                             z = zl + e
                             ok = (0 <= p < lx1) & (y >= 0) & (y <= Y + 1)
+                            if wrap:  # periodic y/z: the interior voxel repeated
+                                y, z, ok = _wrap_at(rows(wv, j), Y), _wrap_at(zl + e, Z), 0 <= p < lx1
                             idx = 4 + p * sX + y * sY + z
                             inb = ok & (idx >= 0) & (idx < len(flat))
                             F[sl, wv, j, :, e] = np.where(inb, flat[np.clip(idx, 0, len(flat) - 1)], 0)
@@ -181,3 +188,40 @@ def test_planted_schedule_mistakes_are_caught(bug):
     got = _tile_sweep(f0, codes, 3, 8, 2, 4, 0, np.float32(o.omega), bug=bug)
     o.run(3)
     assert not np.array_equal(got, o.field.numpy()[0])
+
+
+def _periodic_case(shape, seed):
+    from oracle.restate import OraclePeriodicSolver
+
+    o = OraclePeriodicSolver(_binary(shape, seed))
+    f0 = o.field.numpy()[0].copy()
+    nn = o.factor.numpy()[0]
+    codes = np.zeros(f0.shape, np.int64)
+    codes[1:-1, 1:-1, 1:-1] = np.where(np.isinf(nn), 0, nn).astype(np.int64)
+    return o, f0, codes
+
+
+@pytest.mark.parametrize("shape,H,NW,RPW,TX", [
+    ((5, 6, 8), 3, 8, 2, 4),       # narrow, y halo wraps
+    ((4, 2, 4), 5, 8, 2, 2),       # Y and Z below the halo: several wraps
+    ((4, 20, 140), 4, 4, 4, 2),    # wide + narrow strip
+])
+def test_periodic_wrap_equals_reference_iterations(shape, H, NW, RPW, TX):
+    """Periodic y/z with Y and Z even: H fused half-sweeps equal H periodic
+    reference iterations (ghost wrap before each) on the interior."""
+    o, f0, codes = _periodic_case(shape, sum(shape) + H)
+    got = _tile_sweep(f0, codes, H, NW, RPW, TX, 0, np.float32(o.omega), wrap=True)
+    o.run(H)
+    inner = (slice(1, -1),) * 3
+    assert np.array_equal(got[inner], o.field.numpy()[0][inner])
+
+
+def test_periodic_wrap_needs_even_sizes():
+    """With Z odd the red/black colouring does not repeat across the
+    boundary: the same fused schedule then departs from the reference
+    (which is why the library refuses the periodic flag for odd Y or Z)."""
+    o, f0, codes = _periodic_case((5, 6, 9), 3)
+    got = _tile_sweep(f0, codes, 3, 8, 2, 4, 0, np.float32(o.omega), wrap=True)
+    o.run(3)
+    inner = (slice(1, -1),) * 3
+    assert not np.array_equal(got[inner], o.field.numpy()[0][inner])
