@@ -47,9 +47,6 @@ class ElectrodeSolver(SORSolver):
         self.reac_label = reactive_label
         self.dx = spacing or 1
         super().__init__(img, omega=omega, device=device)
-        B, X = self.batch_size, self.Nx
-        with torch.cuda.device(self.device):
-            self._flux_sum = torch.zeros((B, X), dtype=torch.float64, device=self.device)
         self.c_x = 0
 
     # ---------------- set-up ----------------

@@ -91,9 +91,14 @@ class SORSolver(ABC):
         with torch.cuda.device(self.device):
             self._setup_device()
             self._run_setup()
+            # per-layer float64 sums, flux (B x (X-1), or B x X for the
+            # electrode kind) then concentration (B x X), in one buffer so
+            # that a check copies them to the host in one transfer
             B, X = self.batch_size, self.Nx
-            self._flux_sum = torch.zeros((B, max(X - 1, 1)), dtype=torch.float64, device=self.device)
-            self._conc_sum = torch.zeros((B, X), dtype=torch.float64, device=self.device)
+            self._sums = torch.zeros(2 * B * X, dtype=torch.float64, device=self.device)
+            self._sums_host = torch.empty(2 * B * X, dtype=torch.float64, pin_memory=True)
+        self._sums_ready = None  # event: sums enqueued, not yet read
+        self._ahead = 0  # half-sweeps already launched for the next block
 
         self.converged = False
         self.old_tau = 0
@@ -117,6 +122,8 @@ class SORSolver(ABC):
         with torch.cuda.device(self.device):
             self._run_setup()
         self._cur = 0
+        self._ahead = 0
+        self._sums_ready = None
         self.converged = False
         self.old_tau = 0
         self.iter = 0
@@ -206,20 +213,36 @@ class SORSolver(ABC):
         _lib.check(st, "tf_sweep_launch_config")
         return H, (out.waves, out.rows_per_wave, out.tx), n.value
 
-    def _advance(self, n_half: int) -> None:
-        """Run n_half reference iterations (half-sweeps) on the device."""
-        lib = self._lib
-        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+    def _sweep_args(self):
+        """(per, tile, codes, hm) of the next sweep launches."""
         per, tile = self._launch_config()
         per = max(1, min(per, _lib.MAX_FUSED_HALF_SWEEPS))
         if tile is not None:
             tile = _lib.Tile(*tile)
-        stream = self._stream()
-        # the sweep reads the aligned sweep codes (the per-voxel diffusivity
-        # itself for the field kind)
+        # the sweep reads the aligned sweep codes (the face conductances for
+        # the field kind)
         codes = _lib.ptr(self._sweep_codes if self._sweep_codes is not None else self._codes)
         hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
-        it = self.iter
+        return per, tile, codes, hm
+
+    def _launch(self, h: int, it: int, tile, codes, hm) -> None:
+        """One tf_sweep launch of h half-sweeps from iteration it (its colour)."""
+        B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        src = self._fields[self._cur]
+        dst = self._fields[1 - self._cur]
+        st = self._lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes,
+                                self.kind | self._sweep_flags, B, X, Y, Z, self._w32, it & 1, h,
+                                tile, self._stream())
+        _lib.check(st, "tf_sweep")
+        self._cur = 1 - self._cur
+
+    def _advance(self, n_half: int) -> None:
+        """Run n_half reference iterations (half-sweeps) on the device; the
+        first self._ahead of them were launched ahead of the last check."""
+        done, self._ahead = self._ahead, 0
+        it = self.iter + done
+        n_half -= done
+        per, tile, codes, hm = self._sweep_args()
         # timing (bench.py): HIP events on the launch stream around the
         # launches that fuse the full `per` half-sweeps (a shorter last
         # launch is another kernel and is left out)
@@ -232,13 +255,7 @@ class SORSolver(ABC):
         launches = 0
         while n_half > 0:
             h = min(per, n_half)
-            src = self._fields[self._cur]
-            dst = self._fields[1 - self._cur]
-            st = lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes,
-                              self.kind | self._sweep_flags, B, X, Y, Z, self._w32, it & 1, h,
-                              None if tile is None else tile, stream)
-            _lib.check(st, "tf_sweep")
-            self._cur = 1 - self._cur
+            self._launch(h, it, tile, codes, hm)
             it += h
             n_half -= h
             launches += 1
@@ -246,19 +263,58 @@ class SORSolver(ABC):
                 t1.record()
                 ev.append((t0, t1, full))
 
-    def _layer_sums(self):
-        """Per-layer float64 flux and concentration sums, copied to the host."""
+    def _ahead_room(self, n_next: int) -> int:
+        """Half-sweeps of the next block that may run before its check
+        decision: its first launch (the periodic solvers override)."""
+        return n_next
+
+    def _launch_ahead(self, n_next: int) -> None:
+        """Launch the first sweep of the next block of n_next iterations while
+        the host decides on the check just enqueued.  It reads the checked
+        field and writes the other buffer, so if the solve stops there,
+        _drop_ahead returns to the checked field unchanged."""
+        room = self._ahead_room(n_next)
+        if room <= 0:
+            return
+        per, tile, codes, hm = self._sweep_args()
+        h = min(per, room)
+        self._launch(h, self.iter, tile, codes, hm)
+        self._ahead = h
+
+    def _drop_ahead(self) -> None:
+        if self._ahead:
+            self._cur = 1 - self._cur
+            self._ahead = 0
+
+    def _enqueue_layer_sums(self) -> None:
+        """Layer sums of the current field and their copy to pinned host
+        memory, enqueued on the launch stream; _layer_sums waits for them."""
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        flux = self._sums[:B * X]
+        conc = self._sums[B * X:]
         st = self._lib.tf_layer_sums(
             _lib.ptr(self.field), _lib.ptr(self._codes),
             _lib.ptr(self._hm_lut) if self._hm_lut is not None else None,
             self._n_classes, self.kind, B, X, Y, Z,
-            _lib.ptr(self._flux_sum), _lib.ptr(self._conc_sum), self._stream())
+            _lib.ptr(flux), _lib.ptr(conc), self._stream())
         _lib.check(st, "tf_layer_sums")
+        self._sums_host.copy_(self._sums, non_blocking=True)
+        self._sums_ready = torch.cuda.Event()
+        self._sums_ready.record()
+
+    def _layer_sums(self):
+        """Per-layer float64 flux and concentration sums on the host (those
+        enqueued by _enqueue_layer_sums, else of the current field)."""
+        if self._sums_ready is None:
+            self._enqueue_layer_sums()
+        self._sums_ready.synchronize()
+        self._sums_ready = None
+        B, X = self.batch_size, self.Nx
+        h = self._sums_host.numpy()
         # X - 1 internal faces; the electrode fluxes also have the left face
         width = X if self.kind == _lib.KIND_ELECTRODE else X - 1
-        flux = self._flux_sum.cpu().numpy()[:, :width]
-        conc = self._conc_sum.cpu().numpy()
+        flux = h[:B * width].reshape(B, width).copy()
+        conc = h[B * X:].reshape(B, X).copy()
         return flux, conc
 
     # ---------------- convergence ----------------
@@ -307,14 +363,27 @@ class SORSolver(ABC):
 
         with torch.no_grad(), torch.cuda.device(self.device):
             start = timer()
+            # apply_boundary_conditions before every iteration (reference
+            # loop) is part of _advance: the ghosts of the open solvers are
+            # constant, the periodic solvers wrap (or read across) y and z
             while not self.converged and self.iter < iter_limit:
-                self.apply_boundary_conditions()
                 next_check = (self.iter // CHECK_EVERY + 1) * CHECK_EVERY
                 n = min(next_check, iter_limit) - self.iter
                 self._advance(n)
                 self.iter += n
                 if self.iter % CHECK_EVERY == 0:
-                    self.converged = self.check_convergence(verbose, conv_crit, plot_interval)
+                    # the device runs the first sweep of the next block while
+                    # the host takes the decision; dropped if the solve stops
+                    self._enqueue_layer_sums()
+                    self._launch_ahead(min(self.iter + CHECK_EVERY, iter_limit) - self.iter)
+                    try:
+                        self.converged = self.check_convergence(verbose, conv_crit, plot_interval)
+                    except BaseException:
+                        self._drop_ahead()
+                        self._sums_ready = None
+                        raise
+                    if self.converged:
+                        self._drop_ahead()
             torch.cuda.synchronize(self.device)
             self.walltime = timer() - start
             self._end_simulation(self.iter, verbose)
@@ -876,6 +945,11 @@ class _PeriodicYZ:
         st = self._lib.tf_wrap_ghosts(_lib.ptr(self.field), self.batch_size, self.Nx, self.Ny, self.Nz,
                                       self._stream())
         _lib.check(st, "tf_wrap_ghosts")
+
+    def _ahead_room(self, n_next: int) -> int:
+        # the fused block's last half-sweep runs on its own; unfused, every
+        # half-sweep follows a ghost wrap of the field it reads
+        return n_next - 1 if self._fused_periodic else 0
 
     def _advance(self, n_half: int) -> None:
         if self._fused_periodic:

@@ -216,3 +216,47 @@ def test_solver_z_tiling_boundaries(H, Z, Y):
     s.solve(iter_limit=2 * H + 1, verbose=False)
     o = OracleSolver(img).run(2 * H + 1)
     assert np.array_equal(_field(s), o.field.numpy())
+
+
+@pytest.mark.parametrize("which", ["Solver", "PeriodicSolver", "PeriodicSolver_odd", "MultiPhaseSolver"])
+def test_checks_with_sweep_launched_ahead(which):
+    """At each convergence check the first sweep of the next block is
+    launched before the host's decision and dropped if the solve stops.
+    Checks that do not stop (conv_crit 0), blocks cut by iter_limit, a solve
+    resumed after a check, and a check that stops must all leave the
+    oracle's field."""
+    from oracle.restate import OraclePeriodicSolver
+
+    # odd periodic sizes: the reference's iteration grows without bound
+    # (same-colour neighbours across the boundary); at X = 6 the values stay
+    # finite through 420 iterations, so they still compare exactly
+    shape = (6, 9, 11) if which.endswith("_odd") else (20, 10, 12)
+    if which == "MultiPhaseSolver":
+        img = _labels(shape, n=3, seed=12)
+        Ds = {0: 0.0, 1: 1.0, 2: 0.3}
+        make = lambda: tau.MultiPhaseSolver(img, dict(Ds))  # noqa: E731
+        oracle = lambda: OracleMultiPhaseSolver(img, dict(Ds))  # noqa: E731
+    else:
+        img = _binary(shape, p=0.7, seed=12)
+        cls, ocls = ((tau.Solver, OracleSolver) if which == "Solver" else
+                     (tau.PeriodicSolver, OraclePeriodicSolver))
+        make, oracle = (lambda: cls(img)), (lambda: ocls(img))
+    s = make()
+    s.half_sweeps_per_pass = 6
+    s.solve(iter_limit=250, verbose=False, conv_crit=0.0)   # checks at 100, 200; stops at 250
+    assert s.iter == 250 and not s.converged
+    s.solve(iter_limit=420, verbose=False, conv_crit=0.0)   # resumes; checks at 300, 400
+    # odd Y or Z (periodic): one ghost wrap per half-sweep, whose final
+    # ghosts are a copy taken at another point; compare the interiors
+    cut = (slice(None),) + (slice(1, -1),) * 3 if which.endswith("_odd") else (slice(None),)
+    o = oracle().run(420)
+    assert np.array_equal(_field(s)[cut], o.field.numpy()[cut])
+    if which.endswith("_odd"):
+        return
+    # a solve that stops at a check returns the checked field
+    s2 = make()
+    s2.half_sweeps_per_pass = 5
+    s2.solve(verbose=False)
+    o2 = oracle().solve()
+    assert s2.converged and s2.iter == o2.iter
+    assert np.array_equal(_field(s2)[cut], o2.field.numpy()[cut])
