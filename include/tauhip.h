@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 7
+#define TF_ABI_VERSION 8
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -247,11 +247,31 @@ int tf_sweep_half_inplace(float* field, const uint8_t* codes,
  *            fluxes (taufactor/electrode.py:80-83): c[x-1]-c[x] for x >= 1,
  *            (1 - c[0])/0.5 for x = 0, zeroed where either value is 0
  *   conc_sum device double [B][X]:   sum over (y,z) of c
- * The summation order is fixed, so repeated calls give identical bits.    */
+ * The summation order is fixed (chunks of 2048 positions y*Z + z, each
+ * summed in a fixed tree, then added in chunk order), so repeated calls,
+ * tf_layer_sums_ws and tf_batch_layer_sums give identical bits.          */
 int tf_layer_sums(const float* field, const void* codes,
                   const float* hm_lut, int n_classes, int kind,
                   int B, int X, int Y, int Z,
                   double* flux_sum, double* conc_sum, void* stream);
+
+/* Bytes of device workspace tf_layer_sums_ws needs: 8*B*X*ceil(Y*Z/2048)
+ * doubles (the partial sums of every chunk and wave); negative for invalid
+ * sizes.                                                                 */
+long long tf_layer_sums_workspace_bytes(int B, int X, int Y, int Z);
+
+/* tf_layer_sums with the same result bits, faster on large volumes: each
+ * block walks a slab of x layers of one chunk and reads every plane once
+ * (tf_layer_sums reads the field twice per face), writing the chunk partials
+ * to `workspace`; a second kernel adds them in chunk order.  No reference
+ * counterpart beyond tf_layer_sums'.
+ *   workspace device memory of at least tf_layer_sums_workspace_bytes,
+ *             8-byte aligned; overwritten                                  */
+int tf_layer_sums_ws(const float* field, const void* codes,
+                     const float* hm_lut, int n_classes, int kind,
+                     int B, int X, int Y, int Z,
+                     double* flux_sum, double* conc_sum, void* workspace,
+                     long long workspace_bytes, void* stream);
 
 /* Set-up of a per-voxel diffusivity volume (TF_KIND_FIELD).  No reference
  * counterpart: it is MultiPhaseSolver (taufactor/taufactor.py:585-604) with a

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -56,6 +56,9 @@ SIGNATURES = {
                                        _c_int, _c_float, _c_int, _vp]),
     "tf_layer_sums": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                _vp, _vp, _vp]),
+    "tf_layer_sums_workspace_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int, _c_int]),
+    "tf_layer_sums_ws": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                  _vp, _vp, _vp, ctypes.c_longlong, _vp]),
     "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "tf_setup_field": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_field_faces": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),

@@ -522,17 +522,79 @@ extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int
 // ---------------------------------------------------------------------------
 // per-layer float64 sums for the convergence check
 // ---------------------------------------------------------------------------
-// One block per (x, b).  Each thread accumulates a fixed strided subset of
-// the layer in float64; the block then reduces in a fixed tree, so the result
-// does not depend on scheduling.
-constexpr int kSumThreads = 1024;
+// Summation order (fixed, so repeated calls and every kernel below give the
+// same bits): the Y*Z positions of a layer, flattened as y*Z + z, are cut into
+// chunks of kSumChunk; thread t of 256 adds the terms of positions
+// chunk*kSumChunk + t + 256k, k = 0..kSumPerThread-1, in that order; each wave
+// sums its lanes in a fixed shuffle tree and the four wave sums are added in
+// wave order (the chunk partial); the layer sum is 0 + the chunk partials in
+// chunk order.  tf_layer_sums and tf_batch_layer_sums run one block per layer
+// over its chunks; tf_layer_sums_ws streams x-slabs of one chunk per block,
+// reading each plane once, and adds the partials in a second kernel.
+constexpr int kSumThreads = 256;
+constexpr int kSumPerThread = 8;
+constexpr int kSumChunk = kSumThreads * kSumPerThread;
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
 }
 
-// Sums of layer x of one padded volume V; returns them in thread 0.
+// flux term of a face between padded voxels with values c (x side) and cn
+// (x+1 side), codes k0 and k1: vertical_flux of each solver kind
+// (taufactor/taufactor.py:412-419, 615-620; anisotropic: faces of voxels
+// whose factor exceeds 8 are masked, :417-418)
+template <int KIND>
+__device__ __forceinline__ float face_flux(float c, float cn, code_t<KIND> k0, code_t<KIND> k1,
+                                           const float* __restrict__ hm, int ncls) {
+    float d = cn - c;
+    if constexpr (KIND == TF_KIND_SOLVER) {
+        if (k0 == 0 || k1 == 0) d = 0.0f;
+    } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
+        if (hm[k0] > 8.0f || hm[k1] > 8.0f) d = 0.0f;
+    } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
+        d = hm[k0 * ncls + k1] * d;
+    } else {
+        d = hmean(k0, k1) * d;
+    }
+    return d;
+}
+
+// electrode flux of layer x: -c[x] + c[x-1], (1 - c)/0.5 at x = 0, zero where
+// c or c[x-1] is 0 (ElectrodeSolver.compute_metrics, taufactor/electrode.py:80-83)
+__device__ __forceinline__ float electrode_flux(float c, float cp, int x) {
+    float d = x == 0 ? (1.0f - c) / 0.5f : -c + cp;
+    if (c == 0.0f || cp == 0.0f) d = 0.0f;
+    return d;
+}
+
+// The two chunk partials of a block (accC, accF per thread) into thread 0,
+// in the fixed order.  `slot` alternates between consecutive calls, so one
+// barrier per call suffices.
+__device__ __forceinline__ void chunk_reduce(double accC, double accF, int slot, double& sc,
+                                             double& sf) {
+    __shared__ double red[2][2][kSumThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    accC = wave_sum(accC);
+    accF = wave_sum(accF);
+    if (lane == 0) {
+        red[slot][0][wave] = accC;
+        red[slot][1][wave] = accF;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c = red[slot][0][0], f = red[slot][1][0];
+        for (int i = 1; i < kSumThreads / 64; ++i) {
+            c += red[slot][0][i];
+            f += red[slot][1][i];
+        }
+        sc = c;
+        sf = f;
+    }
+}
+
+// Sums of layer x of one padded volume V, one block looping over the chunks;
+// returns them in thread 0.
 template <int KIND>
 __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
                                                 const code_t<KIND>* __restrict__ Cv,
@@ -540,56 +602,34 @@ __device__ __forceinline__ void layer_sum_block(const float* __restrict__ V,
                                                 const Vol v, int x, double& sum_conc,
                                                 double& sum_flux) {
     const bool has_face = KIND == TF_KIND_ELECTRODE || x < v.X - 1;
-    double accC = 0.0, accF = 0.0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int y = wave; y < v.Y; y += kSumThreads / 64) {
-        const int row = (x + 1) * v.sX + (y + 1) * v.sY + 1;
-        for (int z = lane; z < v.Z; z += 64) {
-            const int o = row + z;
+    const int n = v.Y * v.Z;
+    const int plane = (x + 1) * v.sX;
+    double totC = 0.0, totF = 0.0;
+    for (int ch = 0, slot = 0; ch * kSumChunk < n; ++ch, slot ^= 1) {
+        double accC = 0.0, accF = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSumPerThread; ++k) {
+            const int p = ch * kSumChunk + (int)threadIdx.x + kSumThreads * k;
+            if (p >= n) break;
+            const int y = p / v.Z, z = p - y * v.Z;
+            const int o = plane + (y + 1) * v.sY + z + 1;
             const float c = V[o];
             accC += (double)c;
             if constexpr (KIND == TF_KIND_ELECTRODE) {
-                // fluxes = -c[x] + c[x-1]; at x = 0 (left_bc - c)/0.5; zero
-                // where c or c[x-1] is 0 (taufactor/electrode.py:80-83)
-                const float cp = V[o - v.sX];
-                float d = x == 0 ? (1.0f - c) / 0.5f : -c + cp;
-                if (c == 0.0f || cp == 0.0f) d = 0.0f;
-                accF += (double)d;
+                accF += (double)electrode_flux(c, V[o - v.sX], x);
             } else if (has_face) {
-                const float cn = V[o + v.sX];
-                float d = cn - c;
-                if constexpr (KIND == TF_KIND_SOLVER) {
-                    if (Cv[o] == 0 || Cv[o + v.sX] == 0) d = 0.0f;
-                } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
-                    // Solver.vertical_flux masks faces of voxels whose factor
-                    // exceeds 8 (taufactor/taufactor.py:417-418)
-                    if (hm[Cv[o]] > 8.0f || hm[Cv[o + v.sX]] > 8.0f) d = 0.0f;
-                } else if constexpr (KIND == TF_KIND_MULTIPHASE) {
-                    d = hm[Cv[o] * ncls + Cv[o + v.sX]] * d;
-                } else {
-                    d = hmean(Cv[o], Cv[o + v.sX]) * d;
-                }
-                accF += (double)d;
+                accF += (double)face_flux<KIND>(c, V[o + v.sX], Cv[o], Cv[o + v.sX], hm, ncls);
             }
         }
-    }
-    __shared__ double redC[kSumThreads / 64], redF[kSumThreads / 64];
-    accC = wave_sum(accC);
-    accF = wave_sum(accF);
-    if (lane == 0) {
-        redC[wave] = accC;
-        redF[wave] = accF;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
         double sc = 0.0, sf = 0.0;
-        for (int i = 0; i < kSumThreads / 64; ++i) {
-            sc += redC[i];
-            sf += redF[i];
+        chunk_reduce(accC, accF, slot, sc, sf);
+        if (threadIdx.x == 0) {
+            totC += sc;
+            totF += sf;
         }
-        sum_conc = sc;
-        sum_flux = sf;
     }
+    sum_conc = totC;
+    sum_flux = totF;
 }
 
 template <int KIND>
@@ -598,11 +638,7 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
                                                             const float* __restrict__ hm, int ncls,
                                                             Vol v, double* __restrict__ flux,
                                                             double* __restrict__ conc) {
-    // XCD-aware numbering (gridDim.x is a multiple of 8): blocks x and x+1,
-    // which both read layer x+1, run on the same XCD and share its L2
-    const int per = gridDim.x >> 3;
-    const int x = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3), b = blockIdx.y;
-    if (x >= v.X) return;
+    const int x = blockIdx.x, b = blockIdx.y;
     double sc = 0.0, sf = 0.0;
     layer_sum_block<KIND>(F + (size_t)b * v.sB, C + (size_t)b * v.sB, hm, ncls, v, x, sc, sf);
     if (threadIdx.x == 0) {
@@ -614,35 +650,210 @@ __global__ __launch_bounds__(kSumThreads) void k_layer_sums(const float* __restr
     }
 }
 
-extern "C" int tf_layer_sums(const float* field, const void* codes, const float* hm_lut,
-                             int n_classes, int kind, int B, int X, int Y, int Z, double* flux_sum,
-                             double* conc_sum, void* stream) {
+// Streaming form: block (chunk, x-slab, b) walks the layers [x0, x0 + XS) of
+// one chunk along x, keeping the previous plane's values and codes in
+// registers, so each plane is read once (plus one plane per slab); the loads
+// of the next layer's plane are issued before the current layer's sums.  Each
+// wave writes its tree sum of every layer, part[sel][b*X + x][chunk][wave]
+// (sel 0 concentration, 1 flux); the waves never wait for each other, and
+// k_layer_sums_finish adds the wave sums and chunk partials in the order of
+// chunk_reduce and layer_sum_block.
+template <int KIND>
+__global__ __launch_bounds__(kSumThreads) void k_layer_sums_stream(
+    const float* __restrict__ F, const code_t<KIND>* __restrict__ C, const float* __restrict__ hm,
+    int ncls, Vol v, int XS, double* __restrict__ part) {
+    const int ch = blockIdx.x, nch = gridDim.x, b = blockIdx.z;
+    const int x0 = blockIdx.y * XS, x1 = min(x0 + XS, v.X);
+    if (x0 >= x1) return;
+    const float* V = F + (size_t)b * v.sB;
+    const code_t<KIND>* Cv = C + (size_t)b * v.sB;
+    const int n = v.Y * v.Z;
+    int off[kSumPerThread];
+#pragma unroll
+    for (int k = 0; k < kSumPerThread; ++k) {
+        const int p = ch * kSumChunk + (int)threadIdx.x + kSumThreads * k;
+        const int y = p / v.Z, z = p - y * v.Z;
+        off[k] = p < n ? (y + 1) * v.sY + z + 1 : -1;
+    }
+    // electrode: layer x pairs padded planes x+1 and x; the others x+1, x+2
+    // (for x = X-1 that is the ghost plane X+1, read and not used)
+    constexpr bool kE = KIND == TF_KIND_ELECTRODE;
+    float cq[kSumPerThread], cn[kSumPerThread];
+    code_t<KIND> kq[kSumPerThread], kn[kSumPerThread];
+    auto load = [&](int plane, float* c, code_t<KIND>* kc) {
+#pragma unroll
+        for (int k = 0; k < kSumPerThread; ++k) {
+            const int o = plane * v.sX + max(off[k], 0);
+            c[k] = V[o];
+            kc[k] = kE ? code_t<KIND>(0) : Cv[o];
+        }
+    };
+    load(kE ? x0 : x0 + 1, cq, kq);
+    load(x0 + (kE ? 1 : 2), cn, kn);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int W = kSumThreads / 64;
+    const size_t BX = (size_t)gridDim.z * v.X;
+    double* const pC = part + ((size_t)b * v.X * nch + ch) * W + wave;
+    double* const pF = pC + BX * nch * W;
+    for (int x = x0; x < x1; ++x) {
+        float cn2[kSumPerThread];
+        code_t<KIND> kn2[kSumPerThread];
+        if (x + 1 < x1) load(x + 1 + (kE ? 1 : 2), cn2, kn2);
+        const bool face = kE || x < v.X - 1;
+        double accC = 0.0, accF = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSumPerThread; ++k) {
+            if (off[k] < 0) break;
+            if constexpr (kE) {
+                accC += (double)cn[k];
+                accF += (double)electrode_flux(cn[k], cq[k], x);
+            } else {
+                accC += (double)cq[k];
+                if (face) accF += (double)face_flux<KIND>(cq[k], cn[k], kq[k], kn[k], hm, ncls);
+            }
+        }
+        accC = wave_sum(accC);
+        accF = wave_sum(accF);
+        if (lane == 0) {
+            pC[(size_t)x * nch * W] = accC;
+            pF[(size_t)x * nch * W] = accF;
+        }
+#pragma unroll
+        for (int k = 0; k < kSumPerThread; ++k) {
+            cq[k] = cn[k];
+            kq[k] = kn[k];
+            cn[k] = cn2[k];
+            kn[k] = kn2[k];
+        }
+    }
+}
+
+// layer sums = 0 + the chunk partials in chunk order, each the wave sums in
+// wave order: one 64-thread block per layer stages the partials in LDS and
+// its first lane adds them
+template <int KIND>
+__global__ __launch_bounds__(64) void k_layer_sums_finish(const double* __restrict__ part, int B,
+                                                          int X, int nch, double* __restrict__ flux,
+                                                          double* __restrict__ conc) {
+    constexpr int W = kSumThreads / 64, kStage = 1024;
+    __shared__ double st[2][kStage];
+    const int i = blockIdx.x;
+    const size_t BX = (size_t)B * X;
+    const int n = nch * W;
+    const double* pc = part + (size_t)i * n;
+    const double* pf = part + (BX + i) * (size_t)n;
+    double sc = 0.0, sf = 0.0;
+    for (int base = 0; base < n; base += kStage) {
+        const int m = min(kStage, n - base);
+        for (int j = threadIdx.x; j < m; j += 64) {
+            st[0][j] = pc[base + j];
+            st[1][j] = pf[base + j];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int j = 0; j < m; j += W) {
+                double cc = st[0][j], cf = st[1][j];
+                for (int w = 1; w < W; ++w) {
+                    cc += st[0][j + w];
+                    cf += st[1][j + w];
+                }
+                sc += cc;
+                sf += cf;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int b = i / X, x = i - b * X;
+        conc[i] = sc;
+        if (KIND == TF_KIND_ELECTRODE)
+            flux[i] = sf;
+        else if (x < X - 1)
+            flux[(size_t)b * (X - 1) + x] = sf;
+    }
+}
+
+static int layer_sums_checks(const float* field, const void* codes, const float* hm_lut,
+                             int n_classes, int kind, int B, int X, int Y, int Z,
+                             const double* flux_sum, const double* conc_sum) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
     if (!field || !codes || !conc_sum || (X > 1 && !flux_sum)) return fail(-1, "null pointer argument");
     if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 256))
         return fail(-1, "multiphase sums need hm_lut and 1..256 classes");
+    if (kind == TF_KIND_ANISOTROPIC && (!hm_lut || n_classes != TF_ANISO_TABLE))
+        return fail(-1, "anisotropic sums need the %d-entry prefactor table", TF_ANISO_TABLE);
+    if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD &&
+        kind != TF_KIND_ELECTRODE && kind != TF_KIND_ANISOTROPIC)
+        return fail(-1, "unknown kind %d", kind);
     if (X > 65535 || B > 65535) return fail(-1, "grid too large for layer sums");
+    return 0;
+}
+
+// kernel launch for every kind: L(kernel<KIND>, codes cast to its type)
+template <class L>
+static void by_kind(int kind, const void* codes, L&& launch) {
+    switch (kind) {
+    case TF_KIND_SOLVER: launch(std::integral_constant<int, TF_KIND_SOLVER>{}, static_cast<const uint8_t*>(codes)); break;
+    case TF_KIND_MULTIPHASE: launch(std::integral_constant<int, TF_KIND_MULTIPHASE>{}, static_cast<const uint8_t*>(codes)); break;
+    case TF_KIND_FIELD: launch(std::integral_constant<int, TF_KIND_FIELD>{}, static_cast<const float*>(codes)); break;
+    case TF_KIND_ELECTRODE: launch(std::integral_constant<int, TF_KIND_ELECTRODE>{}, static_cast<const uint8_t*>(codes)); break;
+    default: launch(std::integral_constant<int, TF_KIND_ANISOTROPIC>{}, static_cast<const uint8_t*>(codes)); break;
+    }
+}
+
+extern "C" int tf_layer_sums(const float* field, const void* codes, const float* hm_lut,
+                             int n_classes, int kind, int B, int X, int Y, int Z, double* flux_sum,
+                             double* conc_sum, void* stream) {
+    if (int rc = layer_sums_checks(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum, conc_sum))
+        return rc;
     Vol v = make_vol(X, Y, Z);
     hipStream_t st = (hipStream_t)stream;
-    dim3 grid((X + 7) / 8 * 8, B);
-    if (kind == TF_KIND_SOLVER)
-        hipLaunchKernelGGL(k_layer_sums<TF_KIND_SOLVER>, grid, dim3(kSumThreads), 0, st, field,
-                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
-    else if (kind == TF_KIND_MULTIPHASE)
-        hipLaunchKernelGGL(k_layer_sums<TF_KIND_MULTIPHASE>, grid, dim3(kSumThreads), 0, st, field,
-                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
-    else if (kind == TF_KIND_FIELD)
-        hipLaunchKernelGGL(k_layer_sums<TF_KIND_FIELD>, grid, dim3(kSumThreads), 0, st, field,
-                           static_cast<const float*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
-    else if (kind == TF_KIND_ELECTRODE)
-        hipLaunchKernelGGL(k_layer_sums<TF_KIND_ELECTRODE>, grid, dim3(kSumThreads), 0, st, field,
-                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
-    else if (kind == TF_KIND_ANISOTROPIC && hm_lut && n_classes == TF_ANISO_TABLE)
-        hipLaunchKernelGGL(k_layer_sums<TF_KIND_ANISOTROPIC>, grid, dim3(kSumThreads), 0, st, field,
-                           static_cast<const uint8_t*>(codes), hm_lut, n_classes, v, flux_sum, conc_sum);
-    else
-        return fail(-1, "unknown kind %d", kind);
+    by_kind(kind, codes, [&](auto kc, auto cp) {
+        constexpr int K = decltype(kc)::value;
+        hipLaunchKernelGGL(k_layer_sums<K>, dim3(X, B), dim3(kSumThreads), 0, st, field, cp, hm_lut,
+                           n_classes, v, flux_sum, conc_sum);
+    });
     TF_LAUNCH_CHECK("k_layer_sums");
+    return 0;
+}
+
+static long long layer_sum_chunks(int Y, int Z) {
+    return ((long long)Y * Z + kSumChunk - 1) / kSumChunk;
+}
+
+extern "C" long long tf_layer_sums_workspace_bytes(int B, int X, int Y, int Z) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    return 2LL * B * X * layer_sum_chunks(Y, Z) * (kSumThreads / 64) * (long long)sizeof(double);
+}
+
+extern "C" int tf_layer_sums_ws(const float* field, const void* codes, const float* hm_lut,
+                                int n_classes, int kind, int B, int X, int Y, int Z,
+                                double* flux_sum, double* conc_sum, void* workspace,
+                                long long workspace_bytes, void* stream) {
+    if (int rc = layer_sums_checks(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum, conc_sum))
+        return rc;
+    const long long need = tf_layer_sums_workspace_bytes(B, X, Y, Z);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "layer-sum workspace of %lld bytes needed (got %lld)", need, workspace_bytes);
+    const long long nch = layer_sum_chunks(Y, Z);
+    if (nch > 0x7fffffff) return fail(-1, "too many layer-sum chunks");
+    // x slabs: long enough that the extra plane per slab is a few percent,
+    // short enough for a few waves of blocks per CU
+    int xs = 32;
+    while (xs > 4 && nch * ((X + xs - 1) / xs) * B < 4096) xs >>= 1;
+    Vol v = make_vol(X, Y, Z);
+    hipStream_t st = (hipStream_t)stream;
+    double* part = static_cast<double*>(workspace);
+    const dim3 grid((unsigned)nch, (X + xs - 1) / xs, B);
+    const int fin_blocks = B * X;
+    by_kind(kind, codes, [&](auto kc, auto cp) {
+        constexpr int K = decltype(kc)::value;
+        hipLaunchKernelGGL(k_layer_sums_stream<K>, grid, dim3(kSumThreads), 0, st, field, cp, hm_lut,
+                           n_classes, v, xs, part);
+        hipLaunchKernelGGL(k_layer_sums_finish<K>, dim3(fin_blocks), dim3(64), 0, st, part, B, X,
+                           (int)nch, flux_sum, conc_sum);
+    });
+    TF_LAUNCH_CHECK("k_layer_sums_stream");
     return 0;
 }
 

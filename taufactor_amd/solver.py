@@ -97,6 +97,11 @@ class SORSolver(ABC):
             B, X = self.batch_size, self.Nx
             self._sums = torch.zeros(2 * B * X, dtype=torch.float64, device=self.device)
             self._sums_host = torch.empty(2 * B * X, dtype=torch.float64, pin_memory=True)
+            # per-chunk partial sums of tf_layer_sums_ws
+            ws = self._lib.tf_layer_sums_workspace_bytes(B, X, self.Ny, self.Nz)
+            if ws < 0:
+                _lib.check(int(ws), "tf_layer_sums_workspace_bytes")
+            self._sums_ws = torch.empty(max(ws // 8, 1), dtype=torch.float64, device=self.device)
         self._sums_ready = None  # event: sums enqueued, not yet read
         self._ahead = 0  # half-sweeps already launched for the next block
 
@@ -292,12 +297,13 @@ class SORSolver(ABC):
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
         flux = self._sums[:B * X]
         conc = self._sums[B * X:]
-        st = self._lib.tf_layer_sums(
+        st = self._lib.tf_layer_sums_ws(
             _lib.ptr(self.field), _lib.ptr(self._codes),
             _lib.ptr(self._hm_lut) if self._hm_lut is not None else None,
             self._n_classes, self.kind, B, X, Y, Z,
-            _lib.ptr(flux), _lib.ptr(conc), self._stream())
-        _lib.check(st, "tf_layer_sums")
+            _lib.ptr(flux), _lib.ptr(conc), _lib.ptr(self._sums_ws), self._sums_ws.numel() * 8,
+            self._stream())
+        _lib.check(st, "tf_layer_sums_ws")
         self._sums_host.copy_(self._sums, non_blocking=True)
         self._sums_ready = torch.cuda.Event()
         self._sums_ready.record()

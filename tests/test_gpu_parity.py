@@ -260,3 +260,95 @@ def test_checks_with_sweep_launched_ahead(which):
     o2 = oracle().solve()
     assert s2.converged and s2.iter == o2.iter
     assert np.array_equal(_field(s2)[cut], o2.field.numpy()[cut])
+
+
+def _chunked_sum(terms):
+    """Layer sum in the library's fixed order: chunks of 2048 positions;
+    thread t of 256 adds positions t + 256k in k order; each 64-lane wave
+    sums in a shuffle-down tree; wave sums in wave order; chunk partials in
+    chunk order from 0."""
+    total = 0.0
+    for c0 in range(0, len(terms), 2048):
+        seg = np.zeros(2048)
+        part = terms[c0:c0 + 2048]
+        seg[:len(part)] = part
+        acc = np.zeros(256)
+        for k in range(8):
+            acc = acc + seg[256 * k:256 * (k + 1)]
+        w = acc.reshape(4, 64).copy()
+        for off in (32, 16, 8, 4, 2, 1):
+            w[:, :off] = w[:, :off] + w[:, off:2 * off]
+        chunk = w[0, 0]
+        for i in range(1, 4):
+            chunk = chunk + w[i, 0]
+        total = total + chunk
+    return total
+
+
+def _sums_both_ways(s):
+    from taufactor_amd import _lib
+
+    lib = _lib.load()
+    B, X, Y, Z = s.batch_size, s.Nx, s.Ny, s.Nz
+    hm = _lib.ptr(s._hm_lut) if s._hm_lut is not None else None
+    out = []
+    for ws in (False, True):
+        flux = torch.full((B * X,), np.nan, dtype=torch.float64, device="cuda")
+        conc = torch.full((B * X,), np.nan, dtype=torch.float64, device="cuda")
+        args = (_lib.ptr(s.field), _lib.ptr(s._codes), hm, s._n_classes, s.kind, B, X, Y, Z,
+                _lib.ptr(flux), _lib.ptr(conc))
+        if ws:
+            n = lib.tf_layer_sums_workspace_bytes(B, X, Y, Z)
+            buf = torch.full((n // 8,), np.nan, dtype=torch.float64, device="cuda")
+            st = lib.tf_layer_sums_ws(*args, _lib.ptr(buf), n, _lib.stream_handle(s.device))
+        else:
+            st = lib.tf_layer_sums(*args, _lib.stream_handle(s.device))
+        _lib.check(st, "layer sums")
+        w = X if s.kind == _lib.KIND_ELECTRODE else X - 1
+        out.append((flux.cpu().numpy()[:B * w].reshape(B, w), conc.cpu().numpy().reshape(B, X)))
+    return out
+
+
+@pytest.mark.parametrize("shape", [(17, 19, 23), (5, 50, 90), (40, 64, 64), (70, 3, 700)])
+def test_layer_sums_order_is_pinned(shape):
+    """Per-layer and streaming sums are bit-identical to a numpy evaluation
+    of the documented order (Solver kind, two volumes, several chunks and x
+    slabs, the last chunk partial)."""
+    img = np.stack([_binary(shape, seed=sum(shape)), _binary(shape, p=0.7, seed=1)])
+    s = tau.Solver(img)
+    s.solve(iter_limit=30, verbose=False)
+    (f1, c1), (f2, c2) = _sums_both_ways(s)
+    assert np.array_equal(f1, f2) and np.array_equal(c1, c2)
+    f = s.field.cpu().numpy()
+    k = s._codes.cpu().numpy()
+    X = shape[0]
+    for b in range(2):
+        for x in range(X):
+            c = f[b, x + 1, 1:-1, 1:-1].ravel()
+            assert c1[b, x] == _chunked_sum(c.astype(np.float64))
+            if x < X - 1:
+                cn = f[b, x + 2, 1:-1, 1:-1].ravel()
+                d = (cn - c).astype(np.float32)
+                d[(k[b, x + 1, 1:-1, 1:-1].ravel() == 0) | (k[b, x + 2, 1:-1, 1:-1].ravel() == 0)] = 0
+                assert f1[b, x] == _chunked_sum(d.astype(np.float64))
+
+
+@pytest.mark.parametrize("which", ["multiphase", "electrode", "field", "anisotropic", "periodic"])
+def test_streaming_layer_sums_match_per_layer_sums(which):
+    shape = (37, 41, 57)
+    if which == "multiphase":
+        s = tau.MultiPhaseSolver(_labels(shape, n=4, seed=3), {0: 0.0, 1: 1.0, 2: 0.25, 3: 3.0})
+    elif which == "electrode":
+        rng = np.random.default_rng(5)
+        s = tau.ElectrodeSolver(rng.choice([0, 1, 5], size=shape, p=[0.3, 0.55, 0.15]))
+    elif which == "field":
+        rng = np.random.default_rng(6)
+        s = tau.FieldSolver((rng.random(shape) * (rng.random(shape) < 0.8)).astype(np.float32))
+    elif which == "anisotropic":
+        s = tau.AnisotropicSolver(_binary(shape, p=0.6, seed=7), spacing=(1.0, 1.5, 0.7))
+    else:
+        s = tau.PeriodicSolver(_binary((36, 40, 58), p=0.6, seed=8))
+    s.solve(iter_limit=40, verbose=False)
+    (f1, c1), (f2, c2) = _sums_both_ways(s)
+    assert np.array_equal(f1, f2) and np.array_equal(c1, c2)
+    assert np.isfinite(f1).all() and np.isfinite(c1).all()
