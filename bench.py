@@ -283,7 +283,7 @@ def bench_batch(args) -> int:
     import torch
 
     import taufactor_amd as tau
-    from taufactor_amd import workloads
+    from taufactor_amd import autotune, workloads
 
     t0 = time.perf_counter()
     vols = [v for v, _, _ in workloads.uneven_batch(256, seed=0)]
@@ -293,6 +293,11 @@ def bench_batch(args) -> int:
     b = tau.BatchSolver(vols, device=dev)
     b.solve()  # warm-up (graph capture, autotuned launches)
     torch.cuda.synchronize(dev)
+    for ms, H, tile in autotune.last_timings:
+        log(f"  autotune candidate H={H} tile {tile}: {ms:.4f} ms per half-sweep")
+    t = b._plan["tile"]
+    log(f"batch launch configuration: H={b._plan['H']}, {len(b._plan['schedule'])} passes per "
+        f"100 iterations, tile {(t.waves, t.rows_per_wave, t.tx) if t is not None else 'default'}")
     t0 = time.perf_counter()
     b.reset()
     b.solve()

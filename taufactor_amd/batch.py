@@ -59,7 +59,8 @@ class BatchSolver:
         self._lib = _lib.load()
         self.D_0 = D_0
         # launch configuration: None lets the autotuner (large batches) or the
-        # defaults (H = 4, 16-row tiles) decide; H must divide 100
+        # defaults (H = 4, 16-row tiles) decide; H = 1..6 (a period of 100
+        # iterations ends with a shorter pass when H does not divide 100)
         self.half_sweeps_per_pass = None
         self.tile = None
         self.autotune = True
@@ -141,34 +142,49 @@ class BatchSolver:
             self._setup_all()
         self._plan = None
 
+    @staticmethod
+    def _schedule(H):
+        """Pass depths of one 100-iteration period: passes of H half-sweeps,
+        then one shorter pass for the rest (H = 6: sixteen of 6, one of 4)."""
+        return [H] * (CHECK_EVERY // H) + ([CHECK_EVERY % H] if CHECK_EVERY % H else [])
+
     def _make_plan(self, H, tile):
-        tile = None if tile is None else _lib.Tile(*tile)
-        vols = self._vols_host.copy()
-        tile_vol = []
-        for v, (X, Y, Z) in enumerate(self.shapes):
-            n = ctypes.c_int(0)
-            st = self._lib.tf_sweep_tile_count(X, Y, Z, H, tile, ctypes.byref(n))
-            _lib.check(st, "tf_sweep_tile_count")
-            vols[v]["tile_first"] = len(tile_vol)
-            tile_vol.extend([v] * n.value)
+        """Tile tables of every pass depth of the period (the tiles of a
+        volume depend on the depth), the per-layer volume table and fresh
+        volume states."""
+        tile_c = None if tile is None else _lib.Tile(*tile)
+        dev = self.device
+        depths = {}
+        for h in sorted(set(self._schedule(H))):
+            vols = self._vols_host.copy()
+            tile_vol = []
+            for v, (X, Y, Z) in enumerate(self.shapes):
+                n = ctypes.c_int(0)
+                st = self._lib.tf_sweep_tile_count(X, Y, Z, h, tile_c, ctypes.byref(n))
+                _lib.check(st, "tf_sweep_tile_count")
+                vols[v]["tile_first"] = len(tile_vol)
+                tile_vol.extend([v] * n.value)
+            depths[h] = dict(vols=torch.from_numpy(vols.view(np.uint8).copy()).to(dev),
+                             tile_vol=torch.from_numpy(np.array(tile_vol, dtype=np.int32)).to(dev),
+                             n_tiles=len(tile_vol))
+        # the checks read only the volume geometry, the same in every table
+        self._vols_dev = depths[H]["vols"]
         layer_vol = np.repeat(np.arange(self.n_volumes, dtype=np.int32),
                               [X for (X, _, _) in self.shapes])
-        dev = self.device
-        self._vols_dev = torch.from_numpy(vols.view(np.uint8).copy()).to(dev)
-        self._tile_vol_dev = torch.from_numpy(np.array(tile_vol, dtype=np.int32)).to(dev)
         self._layer_vol_dev = torch.from_numpy(layer_vol).to(dev)
         state = np.zeros(self.n_volumes, dtype=_lib.STATE_DTYPE)
         state["active"] = 1
         self._state_dev = torch.from_numpy(state.view(np.uint8).copy()).to(dev)
-        self._plan = dict(H=H, tile=tile, n_tiles=len(tile_vol))
+        self._plan = dict(H=H, tile=tile_c, depths=depths, schedule=self._schedule(H))
 
     def _pass(self, src_b, colour, h):
         _lib.check(self._pass_status(src_b, colour, h), "tf_batch_sweep")
 
     def _pass_status(self, src_b, colour, h):
         p = self._plan
+        d = p["depths"][h]
         return self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes),
-                                      _lib.ptr(self._vols_dev), _lib.ptr(self._tile_vol_dev), p["n_tiles"],
+                                      _lib.ptr(d["vols"]), _lib.ptr(d["tile_vol"]), d["n_tiles"],
                                       _lib.ptr(self._state_dev), src_b, colour, h, p["tile"], self._stream())
 
     def _check(self, src_b, iter_limit, conv_crit):
@@ -185,12 +201,13 @@ class BatchSolver:
     def _period(self, src_b, iter_limit, conv_crit):
         """100 iterations (passes + check) starting from buffer src_b; returns the
         buffer holding the result."""
-        H = self._plan["H"]
-        for k in range(CHECK_EVERY // H):
-            # a period starts on colour 0 (100 is even); pass k after k*H
-            # half-sweeps
-            self._pass(src_b, (k * H) & 1, H)
+        done = 0
+        for h in self._plan["schedule"]:
+            # a period starts on colour 0 (100 is even); a pass starts on the
+            # colour of the half-sweeps before it
+            self._pass(src_b, done & 1, h)
             src_b ^= 1
+            done += h
         self._check(src_b, iter_limit, conv_crit)
         return src_b
 
@@ -214,8 +231,8 @@ class BatchSolver:
         """Iterate every volume until its own convergence (or iter_limit, a
         multiple of 100).  Returns tau (float32 [V])."""
         H, tile = self._launch_config()
-        if CHECK_EVERY % H or H > _lib.MAX_FUSED_HALF_SWEEPS:
-            raise ValueError("half_sweeps_per_pass must divide 100 (1, 2, 4 or 5)")
+        if not 1 <= H <= _lib.MAX_FUSED_HALF_SWEEPS:
+            raise ValueError(f"half_sweeps_per_pass must be 1..{_lib.MAX_FUSED_HALF_SWEEPS}")
         if iter_limit % CHECK_EVERY:
             raise ValueError("BatchSolver needs iter_limit to be a multiple of 100")
         with torch.no_grad(), torch.cuda.device(self.device):
@@ -225,7 +242,9 @@ class BatchSolver:
             src_b = 0
             done = 0
             graph = None
-            per_graph = 1 if (CHECK_EVERY // H) % 2 == 0 else 2
+            # passes per period: the buffer parity repeats after an even count
+            n_pass = len(self._plan["schedule"])
+            per_graph = 1 if n_pass % 2 == 0 else 2
             while done < n_periods:
                 if graph is None and use_graph and done >= 1 and n_periods - done >= per_graph:
                     graph = torch.cuda.CUDAGraph()
@@ -236,7 +255,7 @@ class BatchSolver:
                 if graph is not None and n_periods - done >= per_graph:
                     graph.replay()
                     done += per_graph
-                    src_b ^= (per_graph * (CHECK_EVERY // H)) & 1
+                    src_b ^= (per_graph * n_pass) & 1
                 else:
                     src_b = self._period(src_b, iter_limit, conv_crit)
                     done += 1

@@ -38,7 +38,7 @@ def _solo(img):
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
-@pytest.mark.parametrize("H", [2, 4, 5])
+@pytest.mark.parametrize("H", [2, 3, 4, 5, 6])
 def test_batch_equals_solo_solves(use_graph, H):
     imgs = _volumes(seed=H)
     b = tau.BatchSolver(imgs)
