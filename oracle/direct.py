@@ -33,16 +33,41 @@ def _hm(a, b):
     return out
 
 
+def _periodic_labels(lab: np.ndarray, n: int) -> np.ndarray:
+    """Merge the 6-connected labels that meet across the y and z wraps."""
+    parent = np.arange(n + 1)
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for a_face, b_face in ((lab[:, 0, :], lab[:, -1, :]), (lab[:, :, 0], lab[:, :, -1])):
+        for a, b in zip(a_face.ravel(), b_face.ravel()):
+            if a and b:
+                ra, rb = find(a), find(b)
+                if ra != rb:
+                    parent[max(ra, rb)] = min(ra, rb)
+    roots = np.array([find(i) for i in range(n + 1)])
+    return roots[lab]
+
+
 def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 40000,
-                      face_scale=(1.0, 1.0, 1.0)):
+                      face_scale=(1.0, 1.0, 1.0), periodic_yz: bool = False):
     """k: float64 [X,Y,Z] voxel diffusivity (0 = non-conductive).
     face_scale: weights of the x, y and z faces (AnisotropicSolver's 1, Ky,
     Kz; the x boundary conductance 2k is not scaled).
+    periodic_yz: y and z wrap round (PeriodicSolver / PeriodicMultiPhaseSolver,
+    taufactor/taufactor.py:481-505, 623-656): the faces between the last and
+    the first layer are faces like any other.
     Returns (c [X,Y,Z] float64, flux_1d [X-1] float64, D_rel float)."""
     k = np.asarray(k, dtype=np.float64)
     X, Y, Z = k.shape
     active = k > 0
     lab, n = scipy.ndimage.label(active, structure=scipy.ndimage.generate_binary_structure(3, 1))
+    if periodic_yz:
+        lab = _periodic_labels(lab, n)
     touching = set(np.unique(lab[0])) | set(np.unique(lab[-1]))
     touching.discard(0)
     keep = np.isin(lab, list(touching)) if touching else np.zeros_like(active)
@@ -56,13 +81,19 @@ def solve_fixed_point(k: np.ndarray, rtol: float = 1e-13, direct_max: int = 4000
     rows, cols, vals = [], [], []
     diag = np.zeros(nun)
     rhs = np.zeros(nun)
+    pairs = []
     for axis in range(3):
         sl_a = [slice(None)] * 3
         sl_b = [slice(None)] * 3
         sl_a[axis] = slice(0, -1)
         sl_b[axis] = slice(1, None)
-        ka, kb = k[tuple(sl_a)], k[tuple(sl_b)]
-        ia, ib = idx[tuple(sl_a)], idx[tuple(sl_b)]
+        pairs.append((axis, tuple(sl_a), tuple(sl_b)))
+        if periodic_yz and axis > 0 and k.shape[axis] > 1:
+            sl_a[axis], sl_b[axis] = slice(-1, None), slice(0, 1)  # the wrap face
+            pairs.append((axis, tuple(sl_a), tuple(sl_b)))
+    for axis, sl_a, sl_b in pairs:
+        ka, kb = k[sl_a], k[sl_b]
+        ia, ib = idx[sl_a], idx[sl_b]
         g = _hm(ka, kb) * face_scale[axis]
         m = (ia >= 0) & (ib >= 0) & (g > 0)
         ga, a_, b_ = g[m], ia[m], ib[m]

@@ -221,3 +221,37 @@ def test_periodic_notebook_table(goldens, which):
     s = cls(img, reduce="f32").solve(conv_crit=1e-3)
     assert s.iter == row["iterations"]
     assert abs(float(s.tau[0]) - row["tau"]) <= 1.5e-6 * row["tau"]
+
+
+def _wrap_channel(X=12, Y=8, Z=6):
+    """A conductive channel that crosses the y boundary halfway along x: it
+    connects the two x faces only across the wrap."""
+    img = np.zeros((X, Y, Z), dtype=np.uint8)
+    img[:X // 2, 0, 2:4] = 1
+    img[X // 2 - 1:, Y - 1, 2:4] = 1
+    return img
+
+
+def test_periodic_direct_solve_matches_periodic_sor():
+    """The periodic float64 direct solve against the periodic SOR oracle run to
+    a tight criterion, on a random two-phase volume and on a channel that
+    percolates only across the wrap (where the closed-boundary solve gives 0)."""
+    from oracle.restate import OraclePeriodicMultiPhaseSolver, OraclePeriodicSolver
+
+    chan = _wrap_channel()
+    assert abs(direct.solver_D_rel(chan)) < 1e-12
+    d_per = direct.solve_fixed_point(chan.astype(np.float64), periodic_yz=True)[2]
+    o = OraclePeriodicSolver(chan).solve(conv_crit=1e-5, iter_limit=20000)
+    assert d_per > 0
+    assert abs(float(o.D_rel[0]) - d_per) <= 1e-3 * d_per
+    rng = np.random.default_rng(3)
+    img = rng.integers(0, 3, size=(10, 8, 8))
+    Ds = {0: 0.0, 1: 1.0, 2: 0.2}
+    k = np.zeros(img.shape)
+    for lbl, d in Ds.items():
+        k[img == lbl] = float(np.float32(d))
+    d_per = direct.solve_fixed_point(k, periodic_yz=True)[2]
+    d_closed = direct.solve_fixed_point(k)[2]
+    o = OraclePeriodicMultiPhaseSolver(img, dict(Ds)).solve(conv_crit=1e-5, iter_limit=20000)
+    assert abs(float(o.D_rel[0]) - d_per) <= 1e-3 * d_per
+    assert abs(d_per - d_closed) > 1e-2 * d_per  # the wrap faces matter here
