@@ -611,8 +611,13 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     float qt = s * rdv.x;
                     const float e2 = __builtin_fmaf(-qt, rdv.y, s);
                     qt = __builtin_fmaf(e2, rdv.x, qt);
-                    if (__builtin_expect(s != 0.0f && __builtin_fabsf(s) < 0x1p-100f, 0))
-                        qt = s / (code ? (float)(code >> 3) : __builtin_inff());
+                    // rare quotients: a wave-uniform test first, so that the
+                    // common path never writes exec (an exec write costs wait
+                    // states before the next DPP add)
+                    const bool rare = s != 0.0f && __builtin_fabsf(s) < 0x1p-100f;
+                    if (__builtin_expect(__builtin_amdgcn_ballot_w64(rare) != 0, 0)) {
+                        if (rare) qt = s / (code ? (float)(code >> 3) : __builtin_inff());
+                    }
                     s = qt - c;
                     s = s * w;
                     out = c + s;

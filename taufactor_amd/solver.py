@@ -56,6 +56,7 @@ class SORSolver(ABC):
     kind: int = _lib.KIND_SOLVER
     _setup_flags: int = 0
     _sweep_flags: int = 0  # OR-ed into the kind of tf_sweep (SWEEP_PERIODIC_YZ)
+    _zero_yz_ghosts: bool = True  # closed y/z boundaries: the ghosts stay 0
 
     def __init__(self, img: np.ndarray, omega: float | None = None, precision=None, device="cuda"):
         self.cpu_img = self._expand_to_4d(img)
@@ -154,7 +155,15 @@ class SORSolver(ABC):
 
     @field.setter
     def field(self, value: torch.Tensor) -> None:
+        """Set both field buffers.  With closed y/z boundaries the y/z ghost
+        layers must hold 0, as the set-up writes them (the reference pads with
+        zeros, taufactor/taufactor.py:227-238): the fused sweep updates halo
+        rows and z voxels unconditionally and relies on a zero ghost staying
+        zero."""
         value = torch.as_tensor(value, dtype=torch.float32, device=self.device)
+        if self._zero_yz_ghosts and value.dim() == 4 and (
+                torch.any(value[:, :, [0, -1], :] != 0) or torch.any(value[:, :, :, [0, -1]] != 0)):
+            raise ValueError("the y/z ghost layers of the field must hold 0 (closed boundaries)")
         for f in self._fields:
             f.copy_(value)
 
@@ -934,6 +943,7 @@ class _PeriodicYZ:
     boundary and the solve alternates one ghost wrap and one half-sweep."""
 
     _setup_flags = _lib.SETUP_PERIODIC_YZ
+    _zero_yz_ghosts = False  # never read: wrapped before use, or read across
 
     @property
     def _fused_periodic(self) -> bool:

@@ -352,3 +352,31 @@ def test_streaming_layer_sums_match_per_layer_sums(which):
     (f1, c1), (f2, c2) = _sums_both_ways(s)
     assert np.array_equal(f1, f2) and np.array_equal(c1, c2)
     assert np.isfinite(f1).all() and np.isfinite(c1).all()
+
+
+@pytest.mark.parametrize("H", [1, 3, 6])
+def test_tiny_and_subnormal_sums_take_the_ieee_division(H):
+    """A field of tiny normal and subnormal values (and zeros, negatives and a
+    few ordinary values): most neighbour sums fall below 2^-100, where the
+    Solver update leaves the reciprocal product for IEEE division.  The
+    iterates must still equal the oracle's, subnormals included."""
+    img = _binary((12, 10, 14), p=0.8, seed=21)
+    rng = np.random.default_rng(5)
+    vals = np.array([0.0, 2.0 ** -110, -(2.0 ** -112), 3e-40, -1e-42, 7e-45, 2.0 ** -101,
+                     -(2.0 ** -126), 0.5], dtype=np.float32)
+    s = tau.Solver(img)
+    o = OracleSolver(img)
+    f = vals[rng.integers(0, len(vals), size=tuple(o.field.shape))].astype(np.float32)
+    bad = f.copy()
+    f[:, :, [0, -1], :] = 0  # closed y/z boundaries: zero ghosts, as the set-up writes
+    f[:, :, :, [0, -1]] = 0
+    with pytest.raises(ValueError):
+        s.field = torch.from_numpy(bad)
+    s.field = torch.from_numpy(f)
+    o.field = torch.from_numpy(f.copy())
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=2 * H + 1, verbose=False)
+    o.run(2 * H + 1)
+    got, want = _field(s), o.field.numpy()
+    assert np.array_equal(got, want)
+    assert np.count_nonzero((np.abs(want) < 2.0 ** -126) & (want != 0)) > 0  # subnormals present
