@@ -614,7 +614,9 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     // rare quotients: a wave-uniform test first, so that the
                     // common path never writes exec (an exec write costs wait
                     // states before the next DPP add)
-                    const bool rare = s != 0.0f && __builtin_fabsf(s) < 0x1p-100f;
+                    // 0 < |s| < 2^-100 <=> frexp exponent <= -100 (0 for zero,
+                    // inf and NaN): one integer compare, no scalar op
+                    const bool rare = __builtin_amdgcn_frexp_expf(s) < -99;
                     if (__builtin_expect(__builtin_amdgcn_ballot_w64(rare) != 0, 0)) {
                         if (rare) qt = s / (code ? (float)(code >> 3) : __builtin_inff());
                     }
