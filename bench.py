@@ -107,8 +107,9 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this solve has no multi-GPU path in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the solve has no multi-GPU path (it would "
+              "shard as x slabs with H-plane ghost slabs exchanged per pass); run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     import numpy as np

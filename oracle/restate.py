@@ -1,8 +1,8 @@
 """Oracle: CPU restatement of the reference's Solver / MultiPhaseSolver solve.
 
 TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py may import this module; it is the checker of
-the HIP path, never part of it.
+cpu_baseline leg of bench.py may import this module; the HIP path's results
+are compared against it, and it is never part of that path.
 
 It restates, op for op in float32 torch on the CPU, what the reference does:
   set-up      taufactor/taufactor.py:24-67 (SORSolver.__init__), 218-253
