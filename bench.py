@@ -241,6 +241,18 @@ def main(argv=None) -> int:
     return 0
 
 
+def _log_launch(s) -> dict:
+    """Log the autotuner's candidates and the resolved launch of solver s."""
+    from taufactor_amd import autotune
+
+    for ms, H, tile in autotune.last_timings:
+        log(f"  autotune candidate H={H} tile {tile}: {ms:.4f} ms per half-sweep")
+    H, tile, n_tiles = s.resolved_launch()
+    log(f"launch configuration: {H} half-sweeps per launch, {tile[0]}x{tile[1]} waves x rows, "
+        f"x tile {tile[2]}, {n_tiles} tiles")
+    return {"half_sweeps_per_pass": H, "tile": list(tile), "tiles_per_launch": n_tiles}
+
+
 def _line(metric, value, unit, steps, warmup, ms, config, higher=True, dtype="f32", **extra):
     out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
            "warmup": warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
@@ -351,6 +363,7 @@ def bench_multiphase(args) -> int:
     Ds = {0: 0.0, 1: 1.0, 2: 0.1}
     s = tau.MultiPhaseSolver(img, dict(Ds), device=dev)
     s.solve(verbose=False)
+    launch = _log_launch(s)
     s._launch_events = []
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
@@ -371,7 +384,7 @@ def bench_multiphase(args) -> int:
     _line("GVoxel-updates/s (MultiPhaseSolver forward) + backward/forward time", round(gvus, 3),
           "GVoxel-updates/s", 1, 1, round(t_fwd * 1e3, 3),
           {"workload": "MultiPhaseSolver 320^3, labels 0/1/2 at 0.4/0.3/0.3 from smoothed noise sigma 3, "
-                       "D {0:0, 1:1.0, 2:0.1}, conv_crit 1e-2", "iterations": s.iter},
+                       "D {0:0, 1:1.0, 2:0.1}, conv_crit 1e-2", "iterations": s.iter, **launch},
           D_eff=float(s.D_eff[0]), tau=float(s.tau[0]),
           grad_D_eff={"1": float(g[1][0]), "2": float(g[2][0])},
           backward_ms=round(t_bwd * 1e3, 3), backward_over_forward=round(t_bwd / t_fwd, 5),
@@ -426,6 +439,7 @@ def bench_field(args) -> int:
     kt = torch.from_numpy(k).to(dev)
     s = tau.FieldSolver(kt, device=dev)
     s.solve(verbose=False)
+    launch = _log_launch(s)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     s.reset()
@@ -443,7 +457,7 @@ def bench_field(args) -> int:
           "GVoxel-updates/s", 1, 1, round(t_fwd * 1e3, 3),
           {"workload": "64 volumes of 128^3, k = sigmoid(4 z), z standardised smoothed noise sigma 3, "
                        "conv_crit 1e-2 (all volumes stop together, reference batch semantics)",
-           "iterations": s.iter},
+           "iterations": s.iter, **launch},
           D_eff_mean=float(s.D_eff.mean()), backward_ms=round(t_bwd * 1e3, 3),
           backward_over_forward=round(t_bwd / t_fwd, 5),
           grad_abs_mean=float(g.abs().mean()))
