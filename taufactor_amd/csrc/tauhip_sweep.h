@@ -351,6 +351,19 @@ __device__ __forceinline__ float from_next_lane(uint32_t v) {
 // halves only at lanes 31 and 32, which are halo lanes of both windows, and
 // the edge rows go through LDS per lane as before; only the addresses and
 // the stores (one per half) differ.
+// Solver kind: the (1/d, d) table in static LDS, whose address is known at
+// compile time (one LDS read per update with no base add, unlike the
+// dynamic area); the other kinds allocate nothing here.
+template <int KIND>
+__device__ __forceinline__ float* solver_rd_table() {
+    if constexpr (KIND == TF_KIND_SOLVER) {
+        __shared__ __align__(16) float t[18];
+        return t;
+    } else {
+        return nullptr;
+    }
+}
+
 template <int H, int KIND, int NW, int RPW>
 __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
                                             const uint8_t* __restrict__ Cg,
@@ -389,8 +402,9 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int zl = ew + 2 * (nar ? (lane & 31) : lane);     // z of the lane's first voxel
     const int lx1 = min(x1 + H, v.X + 2);
 
+    float* const s_rd = solver_rd_table<KIND>();
     if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 18) sT[threadIdx.x] = c_rd[threadIdx.x];
+        if (threadIdx.x < 18) s_rd[threadIdx.x] = c_rd[threadIdx.x];
     } else if (KIND == TF_KIND_ANISOTROPIC) {
         if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
     } else if (KIND == TF_KIND_ELECTRODE) {
@@ -602,7 +616,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                 float out;
                 if constexpr (KIND == TF_KIND_SOLVER) {
                     const float2 rdv =
-                        *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(sT) + code);
+                        *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(s_rd) + code);
                     float s = xp + xm;
                     s = s + yp;
                     s = s + ym;
