@@ -97,6 +97,16 @@ def test_batch_struct_mirrors_match_c_layout():
     assert names == [n for n, _ in Volume._fields_ if n not in ("X", "Y")]  # "int X, Y, Z;"
 
 
+def test_layer_sum_workspace_size():
+    """8 doubles per (volume, layer, 2048-position chunk): the concentration
+    and flux partials of the four waves of a chunk."""
+    lib = _lib.load()
+    for B, X, Y, Z in [(1, 1, 1, 1), (2, 5, 7, 9), (1, 512, 512, 512), (3, 4, 64, 32), (1, 3, 64, 33)]:
+        chunks = -(-(Y * Z) // 2048)
+        assert lib.tf_layer_sums_workspace_bytes(B, X, Y, Z) == 8 * B * X * chunks * 8
+    assert lib.tf_layer_sums_workspace_bytes(0, 1, 1, 1) < 0
+
+
 def test_sweep_code_layout_size():
     lib = _lib.load()
     for X, Y, Z in [(1, 1, 1), (5, 7, 9), (512, 512, 512), (3, 4, 11)]:
