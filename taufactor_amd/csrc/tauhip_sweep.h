@@ -351,13 +351,20 @@ __device__ __forceinline__ float from_next_lane(uint32_t v) {
 // halves only at lanes 31 and 32, which are halo lanes of both windows, and
 // the edge rows go through LDS per lane as before; only the addresses and
 // the stores (one per half) differ.
-// Solver kind: the (1/d, d) table in static LDS, whose address is known at
-// compile time (one LDS read per update with no base add, unlike the
-// dynamic area); the other kinds allocate nothing here.
+// The per-code table of the Solver ((1/d, d) pairs), anisotropic and
+// electrode kinds, in static LDS: its address is known at compile time, so
+// a lookup is one LDS read with no base add (the dynamic area's base is
+// not).  The other kinds allocate nothing here.
 template <int KIND>
-__device__ __forceinline__ float* solver_rd_table() {
+__device__ __forceinline__ float* code_table() {
     if constexpr (KIND == TF_KIND_SOLVER) {
         __shared__ __align__(16) float t[18];
+        return t;
+    } else if constexpr (KIND == TF_KIND_ANISOTROPIC) {
+        __shared__ __align__(16) float t[TF_ANISO_TABLE];
+        return t;
+    } else if constexpr (KIND == TF_KIND_ELECTRODE) {
+        __shared__ __align__(16) float t[TF_ELECTRODE_TABLE];
         return t;
     } else {
         return nullptr;
@@ -402,13 +409,13 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int zl = ew + 2 * (nar ? (lane & 31) : lane);     // z of the lane's first voxel
     const int lx1 = min(x1 + H, v.X + 2);
 
-    float* const s_rd = solver_rd_table<KIND>();
+    float* const s_code = code_table<KIND>();
     if (KIND == TF_KIND_SOLVER) {
-        if (threadIdx.x < 18) s_rd[threadIdx.x] = c_rd[threadIdx.x];
+        if (threadIdx.x < 18) s_code[threadIdx.x] = c_rd[threadIdx.x];
     } else if (KIND == TF_KIND_ANISOTROPIC) {
-        if (threadIdx.x < TF_ANISO_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+        if (threadIdx.x < TF_ANISO_TABLE) s_code[threadIdx.x] = hm[threadIdx.x];
     } else if (KIND == TF_KIND_ELECTRODE) {
-        if (threadIdx.x < TF_ELECTRODE_TABLE) sT[threadIdx.x] = hm[threadIdx.x];
+        if (threadIdx.x < TF_ELECTRODE_TABLE) s_code[threadIdx.x] = hm[threadIdx.x];
     } else if (KIND == TF_KIND_MULTIPHASE) {
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
         // the class ring's pad slots (no wave above the first / below the
@@ -616,7 +623,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                 float out;
                 if constexpr (KIND == TF_KIND_SOLVER) {
                     const float2 rdv =
-                        *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(s_rd) + code);
+                        *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(s_code) + code);
                     float s = xp + xm;
                     s = s + yp;
                     s = s + ym;
@@ -643,7 +650,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     s = s + ym;
                     s = s + zp;
                     s = s + zm;
-                    s = s / sT[code];
+                    s = s / s_code[code];
                     s = s - c;
                     s = s * w;
                     out = c + s;
@@ -685,7 +692,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     t = zp + zm;
                     t = kz * t;
                     s = s + t;
-                    s = s / sT[code];
+                    s = s / s_code[code];
                     s = s - c;
                     s = s * w;
                     out = c + s;
