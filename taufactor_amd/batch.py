@@ -229,7 +229,12 @@ class BatchSolver:
 
     def solve(self, iter_limit=10000, verbose=False, conv_crit=1e-2, use_graph=True):
         """Iterate every volume until its own convergence (or iter_limit, a
-        multiple of 100).  Returns tau (float32 [V])."""
+        multiple of 100).  Returns tau (float32 [V]).  Every call solves from
+        the initial fields: a second call resets them first (volumes that
+        stopped keep their last iterate in either buffer, so continuing from
+        the device state is not an option)."""
+        if self._plan is not None:
+            self.reset()
         H, tile = self._launch_config()
         if not 1 <= H <= _lib.MAX_FUSED_HALF_SWEEPS:
             raise ValueError(f"half_sweeps_per_pass must be 1..{_lib.MAX_FUSED_HALF_SWEEPS}")

@@ -25,6 +25,14 @@ def sha256(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 
 
+def field_sha256(a: np.ndarray) -> str:
+    """SHA-256 of a float32 field's bytes with -0 folded into +0 (x + 0 is +0
+    for x = -0 and x itself otherwise), so that two fields hash alike exactly
+    when np.array_equal holds (NaN-free)."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    return hashlib.sha256((f + np.float32(0)).tobytes()).hexdigest()
+
+
 def _smoothed(rng, shape, sigma):
     z = rng.standard_normal(shape, dtype=np.float32)
     return scipy.ndimage.gaussian_filter(z, sigma, mode="wrap")
