@@ -6,7 +6,8 @@
  * int status (0 = ok, <0 = library error, >0 = hipError_t) and never throws.
  * The message of the last failure on the calling thread is tf_last_error().
  * Device pointers are caller-owned (PyTorch-ROCm tensors); the library does
- * not allocate device memory. `stream` is a hipStream_t passed as void*; all
+ * not allocate device memory (its one device-resident array is the static
+ * scratch of tf_layer_sums_noalloc). `stream` is a hipStream_t passed as void*; all
  * device work is enqueued asynchronously on it.
  *
  * Volume layout (shared by every call):
@@ -36,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 8
+#define TF_ABI_VERSION 9
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -272,6 +273,24 @@ int tf_layer_sums_ws(const float* field, const void* codes,
                      int B, int X, int Y, int Z,
                      double* flux_sum, double* conc_sum, void* workspace,
                      long long workspace_bytes, void* stream);
+
+/* tf_layer_sums_ws without a workspace argument and without allocating:
+ * the partial sums go to a static device array of TF_NOALLOC_SCRATCH_BYTES
+ * that the library holds on every device it runs on until the process ends
+ * (its address is looked up once per device, on the first call there).  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same result bits as tf_layer_sums_ws.
+ * Sizes whose tf_layer_sums_workspace_bytes exceeds TF_NOALLOC_SCRATCH_BYTES
+ * are rejected (-1).  The calling thread's current device must be the device
+ * of `stream`.  Every call on one device shares that device's array: calls
+ * that overlap on one device (different streams, or host threads) race; calls
+ * on one stream are ordered and safe.                                       */
+#define TF_NOALLOC_SCRATCH_BYTES (16LL << 20)
+int tf_layer_sums_noalloc(const float* field, const void* codes,
+                          const float* hm_lut, int n_classes, int kind,
+                          int B, int X, int Y, int Z,
+                          double* flux_sum, double* conc_sum, void* stream);
 
 /* Set-up of a per-voxel diffusivity volume (TF_KIND_FIELD).  No reference
  * counterpart: it is MultiPhaseSolver (taufactor/taufactor.py:585-604) with a

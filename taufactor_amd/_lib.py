@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -24,6 +24,8 @@ ELECTRODE_TABLE = 64
 SETUP_PERIODIC_YZ = 1
 SETUP_ANISOTROPIC = 2
 MAX_FUSED_HALF_SWEEPS = 6
+# size of the static partial-sum array of tf_layer_sums_noalloc (TF_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 16 << 20
 # readable bytes the sweep needs before and after each field buffer (TF_FIELD_SLACK_BYTES)
 FIELD_SLACK_BYTES = 16
 
@@ -59,6 +61,8 @@ SIGNATURES = {
     "tf_layer_sums_workspace_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int, _c_int]),
     "tf_layer_sums_ws": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                                   _vp, _vp, _vp, ctypes.c_longlong, _vp]),
+    "tf_layer_sums_noalloc": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
+                                       _vp, _vp, _vp]),
     "tf_spans_x": (_c_int, [_vp, _c_int, _c_int, _c_int]),
     "tf_setup_field": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_field_faces": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),

@@ -12,6 +12,8 @@
 
 #include "tauhip_sweep.h"
 
+#include <mutex>
+
 // ---------------------------------------------------------------------------
 // error plumbing
 // ---------------------------------------------------------------------------
@@ -826,15 +828,13 @@ extern "C" long long tf_layer_sums_workspace_bytes(int B, int X, int Y, int Z) {
     return 2LL * B * X * layer_sum_chunks(Y, Z) * (kSumThreads / 64) * (long long)sizeof(double);
 }
 
-extern "C" int tf_layer_sums_ws(const float* field, const void* codes, const float* hm_lut,
-                                int n_classes, int kind, int B, int X, int Y, int Z,
-                                double* flux_sum, double* conc_sum, void* workspace,
-                                long long workspace_bytes, void* stream) {
-    if (int rc = layer_sums_checks(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum, conc_sum))
-        return rc;
-    const long long need = tf_layer_sums_workspace_bytes(B, X, Y, Z);
-    if (!workspace || workspace_bytes < need)
-        return fail(-1, "layer-sum workspace of %lld bytes needed (got %lld)", need, workspace_bytes);
+// The two kernels of the streaming layer sums, partials in `part` (every
+// entry the finish kernel reads is written by the stream kernel first, so
+// `part` needs no clearing).
+static int layer_sums_stream_launch(const float* field, const void* codes, const float* hm_lut,
+                                    int n_classes, int kind, int B, int X, int Y, int Z,
+                                    double* flux_sum, double* conc_sum, double* part,
+                                    void* stream) {
     const long long nch = layer_sum_chunks(Y, Z);
     if (nch > 0x7fffffff) return fail(-1, "too many layer-sum chunks");
     // x slabs: long enough that the extra plane per slab is a few percent,
@@ -843,7 +843,6 @@ extern "C" int tf_layer_sums_ws(const float* field, const void* codes, const flo
     while (xs > 4 && nch * ((X + xs - 1) / xs) * B < 4096) xs >>= 1;
     Vol v = make_vol(X, Y, Z);
     hipStream_t st = (hipStream_t)stream;
-    double* part = static_cast<double*>(workspace);
     const dim3 grid((unsigned)nch, (X + xs - 1) / xs, B);
     const int fin_blocks = B * X;
     by_kind(kind, codes, [&](auto kc, auto cp) {
@@ -855,6 +854,71 @@ extern "C" int tf_layer_sums_ws(const float* field, const void* codes, const flo
     });
     TF_LAUNCH_CHECK("k_layer_sums_stream");
     return 0;
+}
+
+extern "C" int tf_layer_sums_ws(const float* field, const void* codes, const float* hm_lut,
+                                int n_classes, int kind, int B, int X, int Y, int Z,
+                                double* flux_sum, double* conc_sum, void* workspace,
+                                long long workspace_bytes, void* stream) {
+    if (int rc = layer_sums_checks(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum, conc_sum))
+        return rc;
+    const long long need = tf_layer_sums_workspace_bytes(B, X, Y, Z);
+    if (!workspace || workspace_bytes < need)
+        return fail(-1, "layer-sum workspace of %lld bytes needed (got %lld)", need, workspace_bytes);
+    return layer_sums_stream_launch(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum,
+                                    conc_sum, static_cast<double*>(workspace), stream);
+}
+
+// ---------------------------------------------------------------------------
+// layer sums without allocation: the partial sums live in a static device
+// array of TF_NOALLOC_SCRATCH_BYTES, one copy per device, held until the
+// process ends.  Its host-side name is not a device address: the address is
+// fetched once per device with hipGetSymbolAddress (that device current) and
+// only that address is handed to kernels.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[TF_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+// device address of `scratch` on the current device (looked up on first use)
+static int scratch_address(hipStream_t st, double** out) {
+    int dev = 0;
+    if (int rc = hip_status(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(-1, "device ordinal %d out of range", dev);
+    if (st) {
+        hipDevice_t sdev = 0;
+        if (int rc = hip_status(hipStreamGetDevice(st, &sdev), "hipStreamGetDevice")) return rc;
+        if ((int)sdev != dev)
+            return fail(-1, "stream belongs to device %d but device %d is current", (int)sdev, dev);
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (int rc = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                                "hipGetSymbolAddress(scratch)"))
+            return rc;
+        if (!p) return fail(-1, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int tf_layer_sums_noalloc(const float* field, const void* codes, const float* hm_lut,
+                                     int n_classes, int kind, int B, int X, int Y, int Z,
+                                     double* flux_sum, double* conc_sum, void* stream) {
+    if (int rc = layer_sums_checks(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum, conc_sum))
+        return rc;
+    const long long need = tf_layer_sums_workspace_bytes(B, X, Y, Z);
+    if (need > TF_NOALLOC_SCRATCH_BYTES)
+        return fail(-1, "tf_layer_sums_noalloc: these sizes need %lld bytes of partial sums, more than "
+                        "the %lld-byte static scratch array (use tf_layer_sums_ws)",
+                    need, (long long)TF_NOALLOC_SCRATCH_BYTES);
+    double* part = nullptr;
+    if (int rc = scratch_address((hipStream_t)stream, &part)) return rc;
+    return layer_sums_stream_launch(field, codes, hm_lut, n_classes, kind, B, X, Y, Z, flux_sum,
+                                    conc_sum, part, stream);
 }
 
 // ---------------------------------------------------------------------------
