@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 9
+#define TF_ABI_VERSION 10
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -60,6 +60,20 @@ extern "C" {
  * for a zero sum). */
 #define TF_KIND_ELECTRODE 4
 #define TF_ELECTRODE_TABLE 64
+/* MultiPhase stencil read through per-voxel face codes (tf_multiphase_face_codes):
+ * `codes` is a uint32 array in the field layout [B][X+2][Y+2][Z+2] with
+ * TF_FIELD_SLACK_BYTES of slack before and after it; bits 0-5, 6-11 and 12-17
+ * hold the byte offsets (4 x table index) of the voxel's x+, y+ and z+ face
+ * conductances in the face table, bits 18-31 the byte offset of its
+ * (1/f, f) prefactor pair in the pair table (1/f correctly rounded).
+ * `hm_lut` holds TF_MPF_FACE_SLOTS face conductances (4 x 4 face classes)
+ * followed by the pairs, pair 0 = (0, 0); n_classes is the table's length in
+ * floats.  Prefactors must lie in [2^-20, 2^20].  Same iterates as
+ * TF_KIND_MULTIPHASE. */
+#define TF_KIND_MULTIPHASE_FACES 5
+#define TF_MPF_MAX_CLASSES 4
+#define TF_MPF_FACE_SLOTS 16
+#define TF_MPF_MAX_FACTORS 2047
 
 /* Workgroup shape of the temporally blocked sweep.  A workgroup streams a
  * tile of (waves*rows_per_wave - 2*n_half) rows in y by TZ voxels in z
@@ -181,6 +195,27 @@ long long tf_sweep_codes_bytes(int X, int Y, int Z);
  * the other kinds the code itself.  */
 int tf_pack_sweep_codes(const uint8_t* codes, uint8_t* sweep_codes, int kind, int B, int X, int Y,
                         int Z, void* stream);
+
+/* Face codes of a labelled volume for tf_sweep with TF_KIND_MULTIPHASE_FACES
+ * (layout at TF_KIND_MULTIPHASE_FACES).  No reference counterpart as an
+ * entry point: it precomputes what MultiPhaseSolver.init_conductive_neighbours
+ * (taufactor/taufactor.py:585-604) derives per voxel - the face
+ * conductances hm(D_a, D_b) and their sum, the prefactor - as table offsets.
+ *   codes       the padded class codes of tf_setup_multiphase
+ *   face_class  device uint8 [n_classes]: the face class (< TF_MPF_MAX_CLASSES,
+ *               one per distinct diffusivity, 0 for diffusivity 0) of each
+ *               class; the face table is 4 x 4, face (a, b) at slot 4a + b
+ *   key_table   device uint16 [4^7 * 4]: the byte offset in the pair table of
+ *               the prefactor of every neighbourhood, indexed by
+ *               (((((((c*4 + xm)*4 + xp)*4 + ym)*4 + yp)*4 + zm)*4 + zp) * 4 + b)
+ *               with c the voxel's face class, xm..zp its neighbours' and
+ *               b = (x == 0) + 2 (x == X-1) the Dirichlet-doubling flags
+ *               (interior 0-based x); offset 0 is the pair (0, 0) of a zero
+ *               (infinite) prefactor
+ *   face_codes  device uint32 padded, written (ghost voxels get pair 0)     */
+int tf_multiphase_face_codes(const uint8_t* codes, const uint8_t* face_class, int n_classes,
+                             const uint16_t* key_table, uint32_t* face_codes, int B, int X, int Y,
+                             int Z, void* stream);
 
 /* n_half over-relaxed red-black half-sweeps (reference iterations), the
  * first one on colour first_colour, fused in one HBM pass: reads src,

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -19,6 +19,10 @@ KIND_FIELD = 2
 KIND_ANISOTROPIC = 3
 ANISO_TABLE = 147
 KIND_ELECTRODE = 4
+KIND_MULTIPHASE_FACES = 5
+MPF_MAX_CLASSES = 4
+MPF_FACE_SLOTS = 16
+MPF_MAX_FACTORS = 2047
 SWEEP_PERIODIC_YZ = 0x100  # flag OR-ed into the kind of tf_sweep
 ELECTRODE_TABLE = 64
 SETUP_PERIODIC_YZ = 1
@@ -52,6 +56,7 @@ SIGNATURES = {
     "tf_wrap_ghosts_from": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep_codes_bytes": (ctypes.c_longlong, [_c_int, _c_int, _c_int]),
     "tf_pack_sweep_codes": (_c_int, [_vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _vp]),
+    "tf_multiphase_face_codes": (_c_int, [_vp, _vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _vp]),
     "tf_sweep": (_c_int, [_vp, _vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,
                           _c_float, _c_int, _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_sweep_half_inplace": (_c_int, [_vp, _vp, _vp, _c_int, _c_int, _c_int, _c_int, _c_int,

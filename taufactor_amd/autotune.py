@@ -68,8 +68,10 @@ def _time(launch, n):
 def tune_solver(s, fixed_h=None, fixed_tile=None):
     """Best (H, tile) for a plain solver `s` (SORSolver subclass)."""
     B, X, Y, Z = s.batch_size, s.Nx, s.Ny, s.Nz
-    kind = s.kind | s._sweep_flags
-    key = ("plain", kind, B, X, Y, Z, fixed_h, None if fixed_tile is None else tuple(fixed_tile))
+    # the kind, codes and table the solver's own launches use (MultiPhase
+    # volumes may sweep through face codes instead of class codes)
+    kind, codes, hm, n_table = s._sweep_pointers()
+    key = ("plain", kind, n_table, B, X, Y, Z, fixed_h, None if fixed_tile is None else tuple(fixed_tile))
     if key in _cache:
         return _cache[key]
     lib = s._lib
@@ -79,8 +81,6 @@ def tune_solver(s, fixed_h=None, fixed_tile=None):
     dst = _lib.slack_buffer(shape, torch.float32, s.device)
     src.copy_(s.field)
     dst.copy_(s.field)
-    codes = _lib.ptr(s._sweep_codes if s._sweep_codes is not None else s._codes)
-    hm = _lib.ptr(s._hm_lut) if s._hm_lut is not None else None
     stream = s._stream()
     timings = []
     for H, tile in _candidates(space, fixed_h, fixed_tile):
@@ -92,7 +92,7 @@ def tune_solver(s, fixed_h=None, fixed_tile=None):
             i = state["i"]
             state["i"] += 1
             return lib.tf_sweep(_lib.ptr(bufs[i & 1]), _lib.ptr(bufs[1 - (i & 1)]), codes, hm,
-                                s._n_classes, kind, B, X, Y, Z, s._w32, 0, H, t, stream)
+                                n_table, kind, B, X, Y, Z, s._w32, 0, H, t, stream)
 
         ms = _time(launch, TIMED_LAUNCHES)
         if ms is not None:

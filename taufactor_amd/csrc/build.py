@@ -16,7 +16,7 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 # the C-ABI and non-sweep kernels, then one unit per sweep kind (compiled in
 # parallel: the sweep templates dominate the build)
-SOURCES = ["tauhip.hip", "sweep_solver.hip", "sweep_multiphase.hip", "sweep_field.hip",
+SOURCES = ["tauhip.hip", "sweep_solver.hip", "sweep_multiphase.hip", "sweep_mpfaces.hip", "sweep_field.hip",
            "sweep_anisotropic.hip", "sweep_electrode.hip", "sweep_batch.hip"]
 INCLUDE = os.path.join(ROOT, "include")
 OUT = os.path.join(PKG, "libtauhip.so")

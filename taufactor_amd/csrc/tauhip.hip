@@ -426,7 +426,8 @@ static int split_sweep_kind(int& kind, int& periodic, int Y, int Z) {
     periodic = (kind & TF_SWEEP_PERIODIC_YZ) ? 1 : 0;
     kind &= 0xff;
     if (periodic) {
-        if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_ELECTRODE)
+        if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_ELECTRODE &&
+            kind != TF_KIND_MULTIPHASE_FACES)
             return fail(-1, "TF_SWEEP_PERIODIC_YZ applies to the solver, multiphase and electrode kinds");
         if ((Y | Z) & 1)
             return fail(-1, "TF_SWEEP_PERIODIC_YZ needs even Y and Z (got %d, %d)", Y, Z);
@@ -439,8 +440,12 @@ static int sweep_prepare(const float* hm_lut, int n_classes, int kind, int B, in
                          int n_half, const tf_tile* tile, tf_tile* t) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
     if (kind != TF_KIND_SOLVER && kind != TF_KIND_MULTIPHASE && kind != TF_KIND_FIELD &&
-        kind != TF_KIND_ANISOTROPIC && kind != TF_KIND_ELECTRODE)
+        kind != TF_KIND_ANISOTROPIC && kind != TF_KIND_ELECTRODE && kind != TF_KIND_MULTIPHASE_FACES)
         return fail(-1, "unknown kind %d", kind);
+    if (kind == TF_KIND_MULTIPHASE_FACES &&
+        (!hm_lut || n_classes < TF_MPF_FACE_SLOTS + 2 || n_classes > TF_MPF_FACE_SLOTS + 2 * TF_MPF_MAX_FACTORS + 2))
+        return fail(-1, "face-code sweep needs a table of %d face slots and 1..%d prefactor pairs (got %d floats)",
+                    TF_MPF_FACE_SLOTS, TF_MPF_MAX_FACTORS + 1, n_classes);
     if (kind == TF_KIND_ELECTRODE && (!hm_lut || n_classes != TF_ELECTRODE_TABLE))
         return fail(-1, "electrode sweep needs the %d-entry prefactor table per volume", TF_ELECTRODE_TABLE);
     if (kind == TF_KIND_MULTIPHASE && (!hm_lut || n_classes < 1 || n_classes > 64))
@@ -470,6 +475,8 @@ static int sweep_dispatch(int kind, int n_half, SweepArgs& a, int B, const tf_ti
         return sweep_dispatch_solver(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_MULTIPHASE)
         return sweep_dispatch_multiphase(n_half, a, B, t.waves, t.rows_per_wave, st);
+    if (kind == TF_KIND_MULTIPHASE_FACES)
+        return sweep_dispatch_mpfaces(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_ANISOTROPIC)
         return sweep_dispatch_anisotropic(n_half, a, B, t.waves, t.rows_per_wave, st);
     if (kind == TF_KIND_ELECTRODE)
@@ -518,6 +525,61 @@ extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int
     out->rows_per_wave = t.rows_per_wave;
     out->tx = a.TX;
     if (n_tiles) *n_tiles = a.total;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// MultiPhase face codes
+// ---------------------------------------------------------------------------
+// One thread per padded voxel of the class codes C (x ghosts copy the
+// adjacent class, y/z ghosts class 0 or, periodic, the opposite layer's),
+// mapped to face classes by fmap: byte offsets 4 (4 a + b) of the faces
+// towards x+1, y+1, z+1 in the 4 x 4 face table (0 where that neighbour is
+// outside the padded volume), and for interior voxels the prefactor's pair
+// offset from the neighbourhood key; ghosts get pair 0.
+__global__ void k_face_codes(const uint8_t* __restrict__ C, const uint8_t* __restrict__ fmap,
+                             const uint16_t* __restrict__ keys, uint32_t* __restrict__ out, Vol v,
+                             int B) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)B * v.sB;
+    if (gid >= total) return;
+    const int b = (int)(gid / v.sB);
+    const int r = (int)(gid - (long long)b * v.sB);
+    const int xp = r / v.sX, r2 = r - xp * v.sX;
+    const int yp = r2 / v.sY, zp = r2 - yp * v.sY;
+    const uint8_t* V = C + (size_t)b * v.sB;
+    auto fc = [&](int o) { return (uint32_t)fmap[V[o]]; };
+    const uint32_t c = fc(r);
+    const uint32_t fx = xp <= v.X ? 4u * (4u * c + fc(r + v.sX)) : 0u;
+    const uint32_t fy = yp <= v.Y ? 4u * (4u * c + fc(r + v.sY)) : 0u;
+    const uint32_t fz = zp <= v.Z ? 4u * (4u * c + fc(r + 1)) : 0u;
+    uint32_t pair = 0;
+    if (xp >= 1 && xp <= v.X && yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z) {
+        uint32_t key = c;
+        key = key * 4 + fc(r - v.sX);
+        key = key * 4 + fc(r + v.sX);
+        key = key * 4 + fc(r - v.sY);
+        key = key * 4 + fc(r + v.sY);
+        key = key * 4 + fc(r - 1);
+        key = key * 4 + fc(r + 1);
+        key = key * 4 + (xp == 1 ? 1u : 0u) + (xp == v.X ? 2u : 0u);
+        pair = keys[key];
+    }
+    out[(size_t)b * v.sB + r] = fx | (fy << 6) | (fz << 12) | (pair << 18);
+}
+
+extern "C" int tf_multiphase_face_codes(const uint8_t* codes, const uint8_t* face_class, int n_classes,
+                                        const uint16_t* key_table, uint32_t* face_codes, int B, int X,
+                                        int Y, int Z, void* stream) {
+    if (int rc = check_dims(B, X, Y, Z)) return rc;
+    if (!codes || !face_class || !key_table || !face_codes) return fail(-1, "null pointer argument");
+    if (n_classes < 1 || n_classes > 256) return fail(-1, "1..256 classes expected (got %d)", n_classes);
+    Vol v = make_vol(X, Y, Z);
+    const long long total = (long long)B * v.sB;
+    const int threads = 256;
+    hipLaunchKernelGGL(k_face_codes, dim3((unsigned)((total + threads - 1) / threads)), dim3(threads), 0,
+                       (hipStream_t)stream, codes, face_class, key_table, face_codes, v, B);
+    TF_LAUNCH_CHECK("k_face_codes");
     return 0;
 }
 

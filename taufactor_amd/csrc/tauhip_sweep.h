@@ -313,12 +313,14 @@ struct RegShape {
     // bytes before the edge buffers: the (1/d, d) or prefactor table, or the
     // ncls x ncls face-conductance table of the MultiPhase kind
     static constexpr int table_bytes(int kind, int ncls) {
-        return kind == TF_KIND_MULTIPHASE ? std::max(TBL, (ncls * ncls * 4 + 15) / 16 * 16) : TBL;
+        return kind == TF_KIND_MULTIPHASE ? std::max(TBL, (ncls * ncls * 4 + 15) / 16 * 16)
+               : kind == TF_KIND_MULTIPHASE_FACES ? 16 : TBL;
     }
     static constexpr size_t lds_bytes(int kind, int ncls = 0) {
         return table_bytes(kind, ncls) + (size_t)H * 2 * EB * sizeof(float) +
                (kind == TF_KIND_FIELD ? (size_t)NR * GEB * sizeof(float) : 0) +
-               (kind == TF_KIND_MULTIPHASE ? (size_t)NR * KEB * sizeof(uint16_t) : 0);
+               (kind == TF_KIND_MULTIPHASE || kind == TF_KIND_MULTIPHASE_FACES
+                    ? (size_t)NR * KEB * sizeof(uint16_t) : 0);
     }
 };
 
@@ -366,6 +368,9 @@ __device__ __forceinline__ float* code_table() {
     } else if constexpr (KIND == TF_KIND_ELECTRODE) {
         __shared__ __align__(16) float t[TF_ELECTRODE_TABLE];
         return t;
+    } else if constexpr (KIND == TF_KIND_MULTIPHASE_FACES) {
+        __shared__ __align__(16) float t[TF_MPF_FACE_SLOTS + 2 * TF_MPF_MAX_FACTORS + 2];
+        return t;
     } else {
         return nullptr;
     }
@@ -385,7 +390,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     // MultiPhase kind: Cg holds the class codes (sweep-code layout), hm the
     // ncls x ncls face-conductance table
     constexpr bool kClasses = KIND == TF_KIND_MULTIPHASE;
+    // MultiPhase face codes: Cg holds one uint32 per voxel in the field
+    // layout (byte offsets of the x+, y+, z+ face conductances and of the
+    // (1/f, f) prefactor pair), hm the face table and the pairs
+    constexpr bool kFaceCodes = KIND == TF_KIND_MULTIPHASE_FACES;
     constexpr int NG = kFaces ? Sh::NR : 1;
+    constexpr int NKF = kFaceCodes ? Sh::NR : 1;
     constexpr int TY = Sh::TY, ML = Sh::ML, NR = Sh::NR, DP = Sh::D, EB = Sh::EB;
     static_assert(TY >= 2 && TY % 2 == 0, "tile rows must be even and positive");
     static_assert(RPW % 2 == 0, "rows per wave must be even");
@@ -416,10 +426,16 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         if (threadIdx.x < TF_ANISO_TABLE) s_code[threadIdx.x] = hm[threadIdx.x];
     } else if (KIND == TF_KIND_ELECTRODE) {
         if (threadIdx.x < TF_ELECTRODE_TABLE) s_code[threadIdx.x] = hm[threadIdx.x];
-    } else if (KIND == TF_KIND_MULTIPHASE) {
+    } else if (KIND == TF_KIND_MULTIPHASE_FACES) {
+        // ncls = the table's length in floats
+        for (int i = threadIdx.x; i < ncls; i += NW * 64) s_code[i] = hm[i];
+    }
+    if (KIND == TF_KIND_MULTIPHASE) {
         for (int i = threadIdx.x; i < ncls * ncls; i += NW * 64) sT[i] = hm[i];
-        // the class ring's pad slots (no wave above the first / below the
-        // last) are read as class codes, i.e. table indices: keep them 0
+    }
+    if (KIND == TF_KIND_MULTIPHASE || KIND == TF_KIND_MULTIPHASE_FACES) {
+        // the class / face ring's pad slots (no wave above the first / below
+        // the last) are read as table indices or offsets: keep them 0
         for (int i = threadIdx.x; i < Sh::NR * 2 * 2 * kRowPairs; i += NW * 64) {
             const int pl = i / (4 * kRowPairs), r = i % (4 * kRowPairs);
             const int slot = r < 2 * kRowPairs ? 0 : NW + 1;
@@ -444,6 +460,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int row_c = sweep_code_row(v.Z);
     const int plane_c = (v.Y + 2) * row_c;
     const rsrc_t rC = make_rsrc(Cg + (size_t)pbase * plane_c, (uint32_t)(lx1 - pbase) * (uint32_t)plane_c);
+    // face codes: one descriptor over the uint32 array, laid out (and
+    // offset) like the field
+    const rsrc_t rK = make_rsrc(reinterpret_cast<const char*>(reinterpret_cast<const uint32_t*>(Cg) +
+                                                             (size_t)pbase * v.sX) -
+                                    kSlackBytes,
+                                span_f + 2u * kSlackBytes);
     rsrc_t rG[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -482,6 +504,7 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     u32x2_t F[NR][RPW];
     uint32_t K[NR][RPW];
     u32x2_t G[NG][RPW][3];  // field kind: gx, gy, gz pairs
+    u32x2_t KF[NKF][RPW];   // face codes of the lane's pair
     // slots the prologue does not load hold planes before the stream start
     // in the first steps: defined values, read by no active half-sweep
 #pragma unroll
@@ -497,6 +520,10 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         for (int j = 0; j < RPW; ++j)
 #pragma unroll
             for (int k = 0; k < 3; ++k) G[i][j][k] = u32x2_t{0u, 0u};
+#pragma unroll
+    for (int i = 0; i < NKF; ++i)
+#pragma unroll
+        for (int j = 0; j < RPW; ++j) KF[i][j] = u32x2_t{0u, 0u};
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         const bool pok = p >= pbase && p < lx1;
@@ -508,6 +535,8 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
                     G[sl][j][k] = __builtin_amdgcn_raw_buffer_load_b64(rG[k], pok ? vf[j] : kOff, sf, 0);
+            } else if constexpr (kFaceCodes) {
+                KF[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rK, pok ? vf[j] : kOff, sf, 0);
             } else {
                 K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
             }
@@ -537,12 +566,19 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     // never read by a later half-sweep of this pass nor stored, and ghosts
     // (value 0 with code 0, whose table entry gives a zero quotient) keep
     // their value) ----
+    // A wave whose rows all lie outside half-sweep h's rows, [h + 1, EY - 1 - h)
+    // of the EY rows from ey0 (the tile grown by H - 1 - h), skips that
+    // half-sweep: its plane count is 0.  The rows it leaves stale are read
+    // only by rows outside the next half-sweep's region (one row narrower at
+    // each end), whose results no owned row reads and no store writes.
     int plo[H], pn[H];
 #pragma unroll
     for (int h = 0; h < H; ++h) {
         const int grow = H - 1 - h;
+        const int r0 = wave * RPW;
+        const bool live = r0 + RPW > h + 1 && r0 < Sh::EY - 1 - h;
         plo[h] = max(x0 - grow, 1);
-        pn[h] = min(x1 + grow, v.X + 1) - plo[h];
+        pn[h] = live ? min(x1 + grow, v.X + 1) - plo[h] : 0;
     }
     // unclipped stream start: with TX and TY even, S0 + ey0 is odd, so the
     // active element of row j at step u of an unrolled block,
@@ -587,6 +623,12 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                 uint16_t* ke = sKE + sl * Sh::KEB + (wave + 1) * 2 * kRowPairs + lane;
                 ke[0] = (uint16_t)K[sl][0];
                 ke[kRowPairs] = (uint16_t)K[sl][RPW - 1];
+            } else if constexpr (kFaceCodes) {
+                // the y+ face offsets of the last row's pair, read as the y-
+                // faces of the first row of the wave below
+                uint16_t* ke = sKE + sl * Sh::KEB + (wave + 1) * 2 * kRowPairs + lane;
+                const u32x2_t k = KF[sl][RPW - 1];
+                ke[kRowPairs] = (uint16_t)(((k.x >> 6) & 63u) | (((k.y >> 6) & 63u) << 8));
             }
         };
 
@@ -606,6 +648,9 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                 const uint16_t* ke = sKE + sl * Sh::KEB + lane;
                 ka = ke[wave * 2 * kRowPairs + kRowPairs];
                 kb = ke[(wave + 2) * 2 * kRowPairs];
+            } else if constexpr (kFaceCodes) {
+                // face codes: the y+ face offsets of the last row above
+                ka = sKE[sl * Sh::KEB + lane + wave * 2 * kRowPairs + kRowPairs];
             }
 #pragma unroll
             for (int j = 0; j < RPW; ++j) {
@@ -671,6 +716,47 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     const Nbr n{c, xp, xm, yp, ym, zp, zm};
                     out = update_multiphase(n, row[cls(K[slp][j])], row[cls(K[slm][j])], row[cls(kyp)],
                                             row[cls(kym)], row[czp], row[czm], p == 1, p == v.X, w);
+                } else if constexpr (kFaceCodes) {
+                    // face conductances at the byte offsets of the codes: the
+                    // voxel's x+, y+, z+ faces and the x-, y-, z- faces as the
+                    // plus faces of the neighbours on the minus side; the
+                    // prefactor f and its correctly rounded reciprocal from
+                    // the pair table (MultiPhaseSolver, taufactor/taufactor.py:
+                    // 598-613, with the sums of :598-604 precomputed)
+                    const char* tb = reinterpret_cast<const char*>(s_code);
+                    auto face = [tb](uint32_t off) { return *reinterpret_cast<const float*>(tb + off); };
+                    const u32x2_t kc = KF[sl][j];
+                    const uint32_t own = a ? kc.y : kc.x;
+                    const uint32_t kxm = a ? KF[slm][j].y : KF[slm][j].x;
+                    const float gxp = face(own & 63u), gyp = face((own >> 6) & 63u);
+                    const float gzp = face((own >> 12) & 63u);
+                    const float gxm = face(kxm & 63u);
+                    const float gym = j > 0 ? face(((a ? KF[sl][j - 1].y : KF[sl][j - 1].x) >> 6) & 63u)
+                                            : face(a ? (ka >> 8) & 63u : ka & 63u);
+                    const uint32_t kzm =
+                        a ? kc.x : (uint32_t)__builtin_amdgcn_update_dpp(0, (int)kc.y, 0x138, 0xf, 0xf, false);
+                    const float gzm = face((kzm >> 12) & 63u);
+                    const float2 rf = *reinterpret_cast<const float2*>(tb + 4 * TF_MPF_FACE_SLOTS + (own >> 18));
+                    float s = xp * gxp + xm * gxm;
+                    s = s + yp * gyp;
+                    s = s + ym * gym;
+                    s = s + zp * gzp;
+                    s = s + zm * gzm;
+                    // s / f: reciprocal product and one residual correction
+                    // (exact for 2^-100 <= |s| with the correctly rounded 1/f
+                    // and f in [2^-20, 2^20]); pair (0, 0) is a zero
+                    // (infinite) prefactor, whose quotient 0 gives the same
+                    // update as s/inf; smaller |s| take the IEEE division
+                    float qt = s * rf.x;
+                    const float e2 = __builtin_fmaf(-qt, rf.y, s);
+                    qt = __builtin_fmaf(e2, rf.x, qt);
+                    const bool rare = __builtin_amdgcn_frexp_expf(s) < -99;
+                    if (__builtin_expect(__builtin_amdgcn_ballot_w64(rare) != 0, 0)) {
+                        if (rare) qt = rf.y != 0.0f ? s / rf.y : 0.0f;
+                    }
+                    s = qt - c;
+                    s = s * w;
+                    out = c + s;
                 } else if constexpr (KIND == TF_KIND_FIELD) {
                     // faces of the voxel: its own gx, gy, gz and those of its
                     // x-1 (previous plane), y-1 (row above) and z-1 (other
@@ -759,7 +845,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     // per-volume codes: the sweep-code layout, or for the field kind the
     // face sections (floats in the padded field layout)
     const uint8_t* codes =
-        KIND == TF_KIND_FIELD
+        KIND == TF_KIND_FIELD || KIND == TF_KIND_MULTIPHASE_FACES
             ? reinterpret_cast<const uint8_t*>(static_cast<const float*>(a.codes) + off)
             : static_cast<const uint8_t*>(a.codes) + (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
@@ -780,6 +866,16 @@ struct SweepGeom {
 template <int H, int NW, int RPW>
 static size_t sweep_lds_bytes(int kind, int ncls) {
     return RegShape<H, NW, RPW>::lds_bytes(kind, ncls);
+}
+
+// LDS of the kind's static table (code_table), with room for its alignment;
+// a launch needs it on top of the dynamic bytes
+static constexpr size_t static_table_bytes(int kind) {
+    return 64 + sizeof(float) * (kind == TF_KIND_SOLVER ? 18
+                                 : kind == TF_KIND_ANISOTROPIC ? TF_ANISO_TABLE
+                                 : kind == TF_KIND_ELECTRODE ? TF_ELECTRODE_TABLE
+                                 : kind == TF_KIND_MULTIPHASE_FACES ? TF_MPF_FACE_SLOTS + 2 * TF_MPF_MAX_FACTORS + 2
+                                 : 0);
 }
 
 // Workgroups of one sweep kernel resident on the current device at once
@@ -839,14 +935,16 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
     } else {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(KIND, a.ncls);
-        if (lds > 160 * 1024)
-            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
+        if (lds + static_table_bytes(KIND) > 160 * 1024)
+            return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H,
+                        lds + static_table_bytes(KIND));
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
             // opt in to more than 64 KiB of dynamic LDS; if the runtime
             // refuses, the launch below reports the error
             if (hipFuncSetAttribute((const void*)k_sweep<H, KIND, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(160 * 1024 - static_table_bytes(KIND))) != hipSuccess)
                 (void)hipGetLastError();
             attr_set = true;
         }
@@ -896,7 +994,8 @@ static int dispatch_shape(SweepArgs& a, int B, int nw, int rpw, hipStream_t st) 
     if constexpr (H <= 4) {
         if (nw == 4 && rpw == 4) return launch_sweep<H, KIND, 4, 4>(a, B, st);
     }
-    if constexpr (KIND == TF_KIND_SOLVER || KIND == TF_KIND_FIELD || KIND == TF_KIND_MULTIPHASE) {
+    if constexpr (KIND == TF_KIND_SOLVER || KIND == TF_KIND_FIELD || KIND == TF_KIND_MULTIPHASE ||
+                  KIND == TF_KIND_MULTIPHASE_FACES) {
         if (nw == 12 && rpw == 2) return launch_sweep<H, KIND, 12, 2>(a, B, st);
     }
     if constexpr (KIND == TF_KIND_SOLVER && H <= 4) {
@@ -987,13 +1086,14 @@ static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
         return fail(-1, "workgroup shape %dx%d has no interior rows for n_half=%d", NW, RPW, H);
     } else {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
-        if (lds > 160 * 1024)
+        if (lds + static_table_bytes(TF_KIND_SOLVER) > 160 * 1024)
             return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
         const int grid = ((a.n_tiles + 7) / 8) * 8;
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
             if (hipFuncSetAttribute((const void*)k_sweep_batch<H, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) != hipSuccess)
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
                 (void)hipGetLastError();
             attr_set = true;
         }
@@ -1021,6 +1121,7 @@ static int dispatch_batch_shape(BatchArgs& a, int nw, int rpw, hipStream_t st) {
 // Launch (or, with a.resolve_only, only size) the sweep of one kind.
 int sweep_dispatch_solver(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);
 int sweep_dispatch_multiphase(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);
+int sweep_dispatch_mpfaces(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);
 int sweep_dispatch_field(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);
 int sweep_dispatch_anisotropic(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);
 int sweep_dispatch_electrode(int H, SweepArgs& a, int B, int nw, int rpw, hipStream_t st);

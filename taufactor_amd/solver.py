@@ -214,6 +214,26 @@ class SORSolver(ABC):
         tile = tuple(tile) if tile is not None else atile
         return H, tile
 
+    def _sweep_target(self):
+        """(kind, codes, table, table length) the sweep reads: this kind's
+        aligned sweep codes (or its codes) and its table."""
+        codes = self._sweep_codes if self._sweep_codes is not None else self._codes
+        return self.kind, codes, self._hm_lut, self._n_classes
+
+    def _sweep_pointers(self):
+        """(kind with the sweep flags, codes address, table address, table
+        length) for tf_sweep, after checking that the codes buffer holds the
+        bytes the kind's layout addresses (the kernel reads it by that layout,
+        whatever the buffer)."""
+        kind, codes, table, n_table = self._sweep_target()
+        need = sweep_codes_bytes(self._lib, kind, self.batch_size, self.Nx, self.Ny, self.Nz)
+        have = codes.numel() * codes.element_size()
+        if have < need:
+            raise RuntimeError(f"sweep codes of kind {kind} need {need} bytes, the buffer holds {have}")
+        if n_table and (table is None or table.numel() < n_table):
+            raise RuntimeError(f"the sweep table of kind {kind} needs {n_table} entries")
+        return kind | self._sweep_flags, _lib.ptr(codes), _lib.ptr(table) if table is not None else None, n_table
+
     def resolved_launch(self):
         """(H, (waves, rows_per_wave, tx), n_tiles) of the next sweep launches
         with every default resolved by the library (tf_sweep_launch_config)."""
@@ -221,7 +241,8 @@ class SORSolver(ABC):
         H = max(1, min(H, _lib.MAX_FUSED_HALF_SWEEPS))
         out, n = _lib.Tile(), ctypes.c_int(0)
         req = _lib.Tile(*tile) if tile is not None else None
-        st = self._lib.tf_sweep_launch_config(self.kind | self._sweep_flags, self._n_classes, self.batch_size, self.Nx,
+        kind, _, _, n_table = self._sweep_pointers()
+        st = self._lib.tf_sweep_launch_config(kind, n_table, self.batch_size, self.Nx,
                                               self.Ny, self.Nz, H, req, ctypes.byref(out),
                                               ctypes.byref(n))
         _lib.check(st, "tf_sweep_launch_config")
@@ -234,9 +255,8 @@ class SORSolver(ABC):
         if tile is not None:
             tile = _lib.Tile(*tile)
         # the sweep reads the aligned sweep codes (the face conductances for
-        # the field kind)
-        codes = _lib.ptr(self._sweep_codes if self._sweep_codes is not None else self._codes)
-        hm = _lib.ptr(self._hm_lut) if self._hm_lut is not None else None
+        # the field kind, the face codes for MultiPhase volumes that fit them)
+        _, codes, hm, _ = self._sweep_pointers()
         return per, tile, codes, hm
 
     def _launch(self, h: int, it: int, tile, codes, hm) -> None:
@@ -244,8 +264,9 @@ class SORSolver(ABC):
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
         src = self._fields[self._cur]
         dst = self._fields[1 - self._cur]
-        st = self._lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, self._n_classes,
-                                self.kind | self._sweep_flags, B, X, Y, Z, self._w32, it & 1, h,
+        kind, _, _, n_table = self._sweep_pointers()
+        st = self._lib.tf_sweep(_lib.ptr(src), _lib.ptr(dst), codes, hm, n_table,
+                                kind, B, X, Y, Z, self._w32, it & 1, h,
                                 tile, self._stream())
         _lib.check(st, "tf_sweep")
         self._cur = 1 - self._cur
@@ -516,6 +537,93 @@ class ThroughTransportSolver(SORSolver):
         return tau, relative_error
 
 
+def sweep_codes_bytes(lib, kind: int, B: int, X: int, Y: int, Z: int) -> int:
+    """Bytes of codes tf_sweep reads for B volumes of the given kind: the
+    sweep-code layout of the one-byte kinds, three float32 face sections of
+    the padded layout for the per-voxel field, one uint32 per padded voxel
+    for the MultiPhase face codes."""
+    padded = B * (X + 2) * (Y + 2) * (Z + 2)
+    if kind == _lib.KIND_FIELD:
+        return 3 * padded * 4
+    if kind == _lib.KIND_MULTIPHASE_FACES:
+        return padded * 4
+    n = lib.tf_sweep_codes_bytes(X, Y, Z)
+    if n < 0:
+        _lib.check(int(n), "tf_sweep_codes_bytes")
+    return B * int(n)
+
+
+def _reciprocal_rn(f: np.ndarray) -> np.ndarray:
+    """1/f rounded correctly to float32 (f: positive normal float32).  The
+    float64 quotient rounded to float32 can be a double rounding away from
+    it, so the neighbours of that value are compared by their exact residual
+    1 - f r (f r has at most 48 significant bits: exact in float64, and so is
+    the difference)."""
+    f64 = f.astype(np.float64)
+    best = (1.0 / f64).astype(np.float32)
+    err = np.abs(1.0 - f64 * best.astype(np.float64))
+    for cand in (np.nextafter(best, np.float32(0)), np.nextafter(best, np.float32(np.inf))):
+        e = np.abs(1.0 - f64 * cand.astype(np.float64))
+        better = e < err
+        best = np.where(better, cand, best)
+        err = np.where(better, e, err)
+    return best.astype(np.float32)
+
+
+def face_code_tables(d_cls: np.ndarray):
+    """Tables of the face-code sweep (TF_KIND_MULTIPHASE_FACES, include/tauhip.h)
+    for the class diffusivities d_cls (float32, class 0 = the zero class), or
+    None when they do not fit it: more than MPF_MAX_CLASSES distinct
+    diffusivities (0 included), more than MPF_MAX_FACTORS distinct
+    prefactors, or a prefactor outside [2^-20, 2^20].
+
+    Returns (face class of each class, uint8; key table, uint16 [4^7 * 4]:
+    byte offset of each neighbourhood's prefactor pair; float32 table: the
+    4 x 4 face conductances hm(D_a, D_b), then the pairs (1/f, f), pair 0 =
+    (0, 0) for a zero prefactor).
+
+    A voxel's prefactor is the reference's float32 sum of its six face
+    conductances, ((((x- + x+) + y-) + y+) + z-) + z+, with the x- face added
+    again in the first layer and the x+ face in the last, infinite where
+    zero (MultiPhaseSolver.init_conductive_neighbours,
+    taufactor/taufactor.py:595-604).  It depends only on the face classes of
+    the voxel and its six neighbours and on those two layer flags, so every
+    such neighbourhood is evaluated here once, with the same float32
+    operations.  The harmonic means follow MultiPhaseSolver._harmonic_mean
+    (:577-583) in float32, as the class-code table does."""
+    M = _lib.MPF_MAX_CLASSES
+    d = np.asarray(d_cls, dtype=np.float32)
+    dvals = np.unique(np.concatenate([np.zeros(1, np.float32), d]))
+    n = len(dvals)
+    if n > M:
+        return None
+    fmap = np.searchsorted(dvals, d).astype(np.uint8)
+    dv = np.zeros(M, np.float32)
+    dv[:n] = dvals
+    a, b = dv[:, None], dv[None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        hm = np.where(a + b > 0, (2 * a * b) / (a + b), np.float32(0)).astype(np.float32)
+    key = np.arange(M ** 7)
+    digit = [(key // M ** (6 - i)) % M for i in range(7)]  # c, xm, xp, ym, yp, zm, zp
+    g = [hm[digit[0], digit[i]] for i in range(1, 7)]
+    f = g[0] + g[1]
+    for k in range(2, 6):
+        f = f + g[k]
+    F = np.stack([f, f + g[0], f + g[1], (f + g[0]) + g[1]], axis=1).reshape(-1)
+    valid = np.repeat(np.all(np.stack(digit) < n, axis=0), 4)
+    vals = np.unique(F[valid & (F != 0)])
+    if len(vals) > _lib.MPF_MAX_FACTORS:
+        return None
+    if vals.size and (vals.min() < 2.0 ** -20 or vals.max() > 2.0 ** 20):
+        return None
+    idx = np.searchsorted(vals, F) + 1
+    idx[(F == 0) | ~valid] = 0
+    keys = (8 * idx).astype(np.uint16)
+    pairs = np.stack([_reciprocal_rn(vals), vals], axis=1).reshape(-1) if vals.size else np.zeros(0, np.float32)
+    table = np.concatenate([hm.reshape(-1), np.zeros(2, np.float32), pairs]).astype(np.float32)
+    return fmap, keys, table
+
+
 def _label_table(img: np.ndarray):
     """(labels, counts[B, L]) of the label values of a [B,X,Y,Z] image."""
     flat = img.reshape(img.shape[0], -1)
@@ -688,6 +796,9 @@ class MultiPhaseSolver(ThroughTransportSolver):
 
     kind = _lib.KIND_MULTIPHASE
     MAX_CLASSES = 64
+    # sweep through per-voxel face codes when the diffusivities allow it
+    # (face_code_tables); False keeps the class-code sweep
+    face_codes = True
 
     def __init__(self, img, diffusivities=None, D_scaling=1, omega=None, device="cuda"):
         if diffusivities is None:
@@ -748,7 +859,23 @@ class MultiPhaseSolver(ThroughTransportSolver):
         self._cls_cond = torch.from_numpy((d_cls > 0).astype(np.uint8)).to(self.device)
         shape = (B, X + 2, Y + 2, Z + 2)
         self._codes = torch.empty(shape, dtype=torch.uint8, device=self.device)
-        self._sweep_codes = _lib.sweep_code_buffer(B, X, Y, Z, self.device)
+        # the face codes serve the fused sweeps (a periodic volume with odd Y
+        # or Z alternates ghost wraps and single half-sweeps on the class codes)
+        fused = getattr(self, "_fused_periodic", True)
+        faces = face_code_tables(d_cls) if self.face_codes and fused else None
+        self._faces = None
+        if faces is not None:
+            cmap, keys, table = faces
+            self._faces = {
+                "cmap": torch.from_numpy(cmap).to(self.device),
+                "keys": torch.from_numpy(keys.view(np.int16)).to(self.device),
+                "table": torch.from_numpy(table).to(self.device),
+                # uint32 codes in the field layout (int32 storage, same bits)
+                "codes": _lib.slack_buffer(shape, torch.int32, self.device),
+            }
+            self._sweep_codes = None
+        else:
+            self._sweep_codes = _lib.sweep_code_buffer(B, X, Y, Z, self.device)
         self._fields = [_lib.slack_buffer(shape, torch.float32, self.device) for _ in range(2)]
         self._count = torch.empty((B, X), dtype=torch.int32, device=self.device)
         self._xinit_dev = self._xinit()
@@ -761,8 +888,21 @@ class MultiPhaseSolver(ThroughTransportSolver):
                                            _lib.ptr(self._fields[1]), _lib.ptr(self._count),
                                            B, X, Y, Z, self._setup_flags, self._stream())
         _lib.check(st, "tf_setup_multiphase")
-        self._pack_sweep_codes()
+        if self._faces is not None:
+            f = self._faces
+            st = self._lib.tf_multiphase_face_codes(_lib.ptr(self._codes), _lib.ptr(f["cmap"]), self._n_classes,
+                                                    _lib.ptr(f["keys"]), _lib.ptr(f["codes"]), B, X, Y, Z,
+                                                    self._stream())
+            _lib.check(st, "tf_multiphase_face_codes")
+        else:
+            self._pack_sweep_codes()
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
+
+    def _sweep_target(self):
+        if self._faces is None:
+            return super()._sweep_target()
+        t = self._faces["table"]
+        return _lib.KIND_MULTIPHASE_FACES, self._faces["codes"], t, t.numel()
 
     def _class_image(self, labels, cls_of_label) -> np.ndarray:
         img = self.cpu_img
