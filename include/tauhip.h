@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 10
+#define TF_ABI_VERSION 11
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -410,6 +410,25 @@ int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* sweep_codes,
                    const tf_volume* vols, const int* tile_vol, int n_tiles,
                    const tf_volume_state* state, int src_buffer, int first_colour,
                    int n_half, const tf_tile* tile, void* stream);
+
+/* Live tile list of a batch pass depth for tf_batch_sweep_queue: writes
+ * live_tiles[0 .. n) = the tiles (indices into tile_vol) of the volumes
+ * still active, in tile order, queue[0] = n, and clears the pass counters
+ * queue[1 .. n_counters].  Run after every convergence check (and once
+ * before the first pass); live_tiles holds n_tiles ints, queue
+ * 1 + n_counters.  No reference counterpart. */
+int tf_batch_compact(const int* tile_vol, int n_tiles, const tf_volume_state* state,
+                     int* live_tiles, int* queue, int n_counters, void* stream);
+
+/* tf_batch_sweep as a work queue: a grid of at most the device's resident
+ * workgroups takes the listed live tiles one at a time from the counter
+ * queue[1 + pass] (each pass of a period its own counter), so tiles of
+ * stopped volumes are not dispatched.  Same results as tf_batch_sweep. */
+int tf_batch_sweep_queue(float* field_a, float* field_b, const uint8_t* sweep_codes,
+                         const tf_volume* vols, const int* tile_vol, const int* live_tiles,
+                         int* queue, int pass, int n_tiles, const tf_volume_state* state,
+                         int src_buffer, int first_colour, int n_half, const tf_tile* tile,
+                         void* stream);
 
 /* Per-layer float64 sums of every active volume (layer_vol[l] = volume of
  * packed layer l; layer x of volume v is l = sum_offset + x).  flux_sum[l]

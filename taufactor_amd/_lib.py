@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -80,6 +80,9 @@ SIGNATURES = {
                                      ctypes.POINTER(ctypes.c_int)]),
     "tf_batch_sweep": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _c_int, _c_int,
                                 ctypes.POINTER(Tile), _vp]),
+    "tf_batch_compact": (_c_int, [_vp, _c_int, _vp, _vp, _vp, _c_int, _vp]),
+    "tf_batch_sweep_queue": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _c_int, _c_int,
+                                      _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_batch_layer_sums": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp]),
     "tf_batch_check": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, ctypes.c_double, _vp]),
 }

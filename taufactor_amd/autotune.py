@@ -125,6 +125,8 @@ def tune_batch(b, fixed_h=None, fixed_tile=None):
             b._make_plan(H, tile)
 
             def launch():
+                # the full pass over every tile (a queue pass would find its
+                # counter spent after the first of the timed launches)
                 return b._pass_status(0, 0, H)
 
             ms = _time(launch, TIMED_LAUNCHES)

@@ -64,6 +64,10 @@ class BatchSolver:
         self.half_sweeps_per_pass = None
         self.tile = None
         self.autotune = True
+        # passes as a work queue over the live tiles (tf_batch_sweep_queue);
+        # False dispatches every tile of every volume and lets the tiles of
+        # stopped volumes return at once (tf_batch_sweep)
+        self.use_queue = True
         self.shapes = [tuple(int(d) for d in i.shape) for i in imgs]
         V = self.n_volumes = len(imgs)
         vols = np.zeros(V, dtype=_lib.VOLUME_DTYPE)
@@ -166,7 +170,10 @@ class BatchSolver:
                 tile_vol.extend([v] * n.value)
             depths[h] = dict(vols=torch.from_numpy(vols.view(np.uint8).copy()).to(dev),
                              tile_vol=torch.from_numpy(np.array(tile_vol, dtype=np.int32)).to(dev),
-                             n_tiles=len(tile_vol))
+                             n_tiles=len(tile_vol),
+                             # live tile list and [live count, one counter per pass]
+                             live=torch.zeros(len(tile_vol), dtype=torch.int32, device=dev),
+                             queue=torch.zeros(1 + len(self._schedule(H)), dtype=torch.int32, device=dev))
         # the checks read only the volume geometry, the same in every table
         self._vols_dev = depths[H]["vols"]
         layer_vol = np.repeat(np.arange(self.n_volumes, dtype=np.int32),
@@ -176,13 +183,30 @@ class BatchSolver:
         state["active"] = 1
         self._state_dev = torch.from_numpy(state.view(np.uint8).copy()).to(dev)
         self._plan = dict(H=H, tile=tile_c, depths=depths, schedule=self._schedule(H))
+        self._compact()
 
-    def _pass(self, src_b, colour, h):
-        _lib.check(self._pass_status(src_b, colour, h), "tf_batch_sweep")
+    def _compact(self):
+        """Live tile lists of every pass depth from the current volume states,
+        and the pass counters cleared (after each check)."""
+        n_counters = len(self._plan["schedule"])
+        for d in self._plan["depths"].values():
+            st = self._lib.tf_batch_compact(_lib.ptr(d["tile_vol"]), d["n_tiles"], _lib.ptr(self._state_dev),
+                                            _lib.ptr(d["live"]), _lib.ptr(d["queue"]), n_counters, self._stream())
+            _lib.check(st, "tf_batch_compact")
 
-    def _pass_status(self, src_b, colour, h):
+    def _pass(self, src_b, colour, h, index=0):
+        _lib.check(self._pass_status(src_b, colour, h, index, self.use_queue), "tf_batch_sweep")
+
+    def _pass_status(self, src_b, colour, h, index=0, queue=False):
+        """One pass of depth h from buffer src_b; with queue, the work-queue form
+        on the live tiles, counting with the counter of pass `index`."""
         p = self._plan
         d = p["depths"][h]
+        if queue:
+            return self._lib.tf_batch_sweep_queue(
+                _lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes), _lib.ptr(d["vols"]),
+                _lib.ptr(d["tile_vol"]), _lib.ptr(d["live"]), _lib.ptr(d["queue"]), index, d["n_tiles"],
+                _lib.ptr(self._state_dev), src_b, colour, h, p["tile"], self._stream())
         return self._lib.tf_batch_sweep(_lib.ptr(self._fa), _lib.ptr(self._fb), _lib.ptr(self._sweep_codes),
                                       _lib.ptr(d["vols"]), _lib.ptr(d["tile_vol"]), d["n_tiles"],
                                       _lib.ptr(self._state_dev), src_b, colour, h, p["tile"], self._stream())
@@ -202,13 +226,15 @@ class BatchSolver:
         """100 iterations (passes + check) starting from buffer src_b; returns the
         buffer holding the result."""
         done = 0
-        for h in self._plan["schedule"]:
+        for i, h in enumerate(self._plan["schedule"]):
             # a period starts on colour 0 (100 is even); a pass starts on the
             # colour of the half-sweeps before it
-            self._pass(src_b, done & 1, h)
+            self._pass(src_b, done & 1, h, i)
             src_b ^= 1
             done += h
         self._check(src_b, iter_limit, conv_crit)
+        if self.use_queue:
+            self._compact()
         return src_b
 
     def _launch_config(self):

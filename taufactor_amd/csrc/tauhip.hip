@@ -1279,12 +1279,82 @@ extern "C" int tf_batch_sweep(float* field_a, float* field_b, const uint8_t* cod
         return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
-    BatchArgs a;
+    BatchArgs a = {};
     a.fa = field_a; a.fb = field_b; a.codes = codes; a.vols = vols; a.tile_vol = tile_vol;
     a.state = state; a.n_tiles = n_tiles; a.src_b = src_buffer; a.colour0 = first_colour;
     a.TX = t.tx;
     hipStream_t st = (hipStream_t)stream;
     return sweep_dispatch_batch(n_half, a, t.waves, t.rows_per_wave, st);
+}
+
+// Live tile list of one pass depth: the tiles of active volumes in tile
+// order (one block; chunks of 1024 tiles, a prefix sum of the active flags
+// per chunk), their count in queue[0] and the pass counters queue[1..]
+// cleared.
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_batch_compact(
+    const int* __restrict__ tile_vol, int n_tiles, const tf_volume_state* __restrict__ state,
+    int* __restrict__ live, int* __restrict__ queue, int n_counters) {
+    __shared__ int s_scan[kCompactThreads];
+    __shared__ int s_base;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_base = 0;
+    __syncthreads();
+    for (int c0 = 0; c0 < n_tiles; c0 += kCompactThreads) {
+        const int t = c0 + tid;
+        const int f = (t < n_tiles && state[tile_vol[t]].active) ? 1 : 0;
+        s_scan[tid] = f;
+        __syncthreads();
+        for (int off = 1; off < kCompactThreads; off <<= 1) {  // inclusive scan
+            const int v = tid >= off ? s_scan[tid - off] : 0;
+            __syncthreads();
+            s_scan[tid] += v;
+            __syncthreads();
+        }
+        if (f) live[s_base + s_scan[tid] - 1] = t;
+        __syncthreads();
+        if (tid == kCompactThreads - 1) s_base += s_scan[tid];
+        __syncthreads();
+    }
+    if (tid == 0) queue[0] = s_base;
+    for (int i = tid; i < n_counters; i += kCompactThreads) queue[1 + i] = 0;
+}
+
+extern "C" int tf_batch_compact(const int* tile_vol, int n_tiles, const tf_volume_state* state,
+                                int* live_tiles, int* queue, int n_counters, void* stream) {
+    if (!tile_vol || !state || !live_tiles || !queue) return fail(-1, "null pointer argument");
+    if (n_tiles < 1 || n_tiles >= (1 << 28)) return fail(-1, "n_tiles=%d out of range", n_tiles);
+    if (n_counters < 0 || n_counters > (1 << 20)) return fail(-1, "n_counters=%d out of range", n_counters);
+    hipLaunchKernelGGL(k_batch_compact, dim3(1), dim3(kCompactThreads), 0, (hipStream_t)stream, tile_vol,
+                       n_tiles, state, live_tiles, queue, n_counters);
+    TF_LAUNCH_CHECK("k_batch_compact");
+    return 0;
+}
+
+extern "C" int tf_batch_sweep_queue(float* field_a, float* field_b, const uint8_t* codes,
+                                    const tf_volume* vols, const int* tile_vol, const int* live_tiles,
+                                    int* queue, int pass, int n_tiles, const tf_volume_state* state,
+                                    int src_buffer, int first_colour, int n_half, const tf_tile* tile,
+                                    void* stream) {
+    if (!field_a || !field_b || !codes || !vols || !tile_vol || !state || !live_tiles || !queue)
+        return fail(-1, "null pointer argument");
+    if (field_a == field_b) return fail(-1, "field_a and field_b must be distinct buffers");
+    if (n_tiles < 1 || n_tiles >= (1 << 28)) return fail(-1, "n_tiles=%d out of range", n_tiles);
+    if (pass < 0 || pass >= (1 << 20)) return fail(-1, "pass=%d out of range", pass);
+    if (first_colour != 0 && first_colour != 1) return fail(-1, "first_colour must be 0 or 1");
+    if (src_buffer != 0 && src_buffer != 1) return fail(-1, "src_buffer must be 0 or 1");
+    if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS)
+        return fail(-1, "n_half=%d out of range 1..%d", n_half, TF_MAX_FUSED_HALF_SWEEPS);
+    tf_tile t;
+    if (int rc = resolve_tile(n_half, tile, &t)) return rc;
+    BatchArgs a = {};
+    a.fa = field_a; a.fb = field_b; a.codes = codes; a.vols = vols; a.tile_vol = tile_vol;
+    a.state = state; a.n_tiles = n_tiles; a.src_b = src_buffer; a.colour0 = first_colour;
+    a.TX = t.tx;
+    a.live = live_tiles;
+    a.queue = queue;
+    a.pass = pass;
+    return sweep_dispatch_batch(n_half, a, t.waves, t.rows_per_wave, (hipStream_t)stream);
 }
 
 __global__ __launch_bounds__(kSumThreads) void k_batch_layer_sums(

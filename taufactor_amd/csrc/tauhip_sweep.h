@@ -102,18 +102,19 @@ __device__ __forceinline__ void st_f32(float val, rsrc_t r, int byte_off, int so
 // exact float32 division by a conductive-neighbour count
 // ---------------------------------------------------------------------------
 // q = s / d for d in {inf, 1..8}, correctly rounded (the value IEEE division
-// gives).  One reciprocal product plus one residual correction with the
-// correctly rounded reciprocal r = RN(1/d); numerators below 2^-100 (where
-// the correction can be off for d = 6, and where the sign of zero matters)
-// take the IEEE division.  Exhaustively checked against s/d over all finite
-// float32 s for each d (tests/test_division.py samples that check).
+// gives, a zero possibly of the other sign, which gives the same update).
+// One reciprocal product plus one residual correction with the correctly
+// rounded reciprocal r = RN(1/d) (r = 0 and a zero divisor stand for inf);
+// the correction is exact unless the residual is subnormal, and those
+// quotients take the IEEE division.  Exhaustively checked against s/d over
+// all float32 s for each d (tests/csrc/divcheck.c, run by
+// tests/test_division.py).
 __device__ __forceinline__ float div_by_count(float s, float r, float d, int code) {
+    const float dd = code != 0 ? d : 0.0f;
     float q = s * r;
-    if (code != 0) {
-        float e = __builtin_fmaf(-q, d, s);
-        q = __builtin_fmaf(e, r, q);
-    }
-    if (__builtin_expect(__builtin_fabsf(s) < 0x1p-100f, 0)) q = s / d;
+    const float e = __builtin_fmaf(-q, dd, s);
+    q = __builtin_fmaf(e, r, q);
+    if (__builtin_amdgcn_classf(e, 0x90)) q = s / d;
     return q;
 }
 
@@ -679,10 +680,11 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     qt = __builtin_fmaf(e2, rdv.x, qt);
                     // rare quotients: a wave-uniform test first, so that the
                     // common path never writes exec (an exec write costs wait
-                    // states before the next DPP add)
-                    // 0 < |s| < 2^-100 <=> frexp exponent <= -100 (0 for zero,
-                    // inf and NaN): one integer compare, no scalar op
-                    const bool rare = __builtin_amdgcn_frexp_expf(s) < -99;
+                    // states before the next DPP add).  The correction is
+                    // exact unless the residual e2 is subnormal (then some of
+                    // its bits are lost; tests/csrc/divcheck.c checks every
+                    // float32 s): one class test of e2
+                    const bool rare = __builtin_amdgcn_classf(e2, 0x90);
                     if (__builtin_expect(__builtin_amdgcn_ballot_w64(rare) != 0, 0)) {
                         if (rare) qt = s / (code ? (float)(code >> 3) : __builtin_inff());
                     }
@@ -744,7 +746,10 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                     s = s + zm * gzm;
                     // s / f: reciprocal product and one residual correction
                     // (exact for 2^-100 <= |s| with the correctly rounded 1/f
-                    // and f in [2^-20, 2^20]); pair (0, 0) is a zero
+                    // and f in [2^-20, 2^20], tests/csrc/divcheck_pairs.c;
+                    // unlike the integer counts of the Solver kind, a
+                    // subnormal-range s can leave a residual that rounds to
+                    // zero, so the guard is on s); pair (0, 0) is a zero
                     // (infinite) prefactor, whose quotient 0 gives the same
                     // update as s/inf; smaller |s| take the IEEE division
                     float qt = s * rf.x;
@@ -1040,14 +1045,17 @@ struct BatchArgs {
     int src_b;
     int colour0;
     int TX;
+    // work queue (tf_batch_sweep_queue): live tile list and counters, or null
+    const int* live;
+    int* queue;
+    int pass;
 };
 
+// Tile `logical` of the batch's tile list (every tile of every volume, in
+// volume order) for the active volumes; returns at once for the others.
 template <int H, int NW, int RPW>
-__global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
+__device__ __forceinline__ void batch_tile(const BatchArgs& a, int logical) {
     using Sh = RegShape<H, NW, RPW>;
-    const int per = gridDim.x >> 3;
-    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    if (logical >= a.n_tiles) return;
     const int vi = a.tile_vol[logical];
     if (!a.state[vi].active) return;
     const tf_volume vol = a.vols[vi];
@@ -1080,6 +1088,34 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
 }
 
 template <int H, int NW, int RPW>
+__global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
+    const int per = gridDim.x >> 3;
+    const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (logical >= a.n_tiles) return;
+    batch_tile<H, NW, RPW>(a, logical);
+}
+
+// The same pass as a work queue: a grid of at most the resident workgroups
+// takes the live tiles (a.live[0 .. a.queue[0]), listed by k_batch_compact
+// after the last check) one at a time from the pass's counter
+// a.queue[1 + a.pass], which k_batch_compact zeroed.  Every workgroup leaves
+// once the counter has passed the live count, so tiles of stopped volumes
+// are never dispatched and no workgroup waits for another.
+template <int H, int NW, int RPW>
+__global__ __launch_bounds__(NW * 64) void k_sweep_batch_queue(BatchArgs a) {
+    __shared__ int s_next;
+    const int n_live = a.queue[0];
+    for (;;) {
+        if (threadIdx.x == 0) s_next = atomicAdd(&a.queue[1 + a.pass], 1);
+        __syncthreads();
+        const int t = s_next;
+        __syncthreads();  // every wave has read s_next before it is written again
+        if (t >= n_live) break;
+        batch_tile<H, NW, RPW>(a, a.live[t]);
+    }
+}
+
+template <int H, int NW, int RPW>
 static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
     using Sh = RegShape<H, NW, RPW>;
     if constexpr (Sh::TY < 2) {
@@ -1088,16 +1124,30 @@ static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
         if (lds + static_table_bytes(TF_KIND_SOLVER) > 160 * 1024)
             return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
-        const int grid = ((a.n_tiles + 7) / 8) * 8;
+        int grid = ((a.n_tiles + 7) / 8) * 8;
+        if (a.queue) {
+            // the queue form: at most the resident workgroups
+            static int slots_cache[kMaxDevices] = {};
+            const int slots = resident_workgroups((const void*)k_sweep_batch_queue<H, NW, RPW>, NW * 64,
+                                                  lds, slots_cache);
+            grid = slots > 0 ? std::min(a.n_tiles, slots) : std::min(a.n_tiles, 1024);
+        }
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
+            if (hipFuncSetAttribute((const void*)k_sweep_batch_queue<H, NW, RPW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
+                (void)hipGetLastError();
             if (hipFuncSetAttribute((const void*)k_sweep_batch<H, NW, RPW>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
                 (void)hipGetLastError();
             attr_set = true;
         }
-        hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        if (a.queue)
+            hipLaunchKernelGGL((k_sweep_batch_queue<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
         TF_LAUNCH_CHECK("k_sweep_batch");
         return 0;
     }

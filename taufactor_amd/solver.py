@@ -795,7 +795,11 @@ class MultiPhaseSolver(ThroughTransportSolver):
     """
 
     kind = _lib.KIND_MULTIPHASE
+    # classes (labels + the zero ghost class) of the class-code sweep and of
+    # grad_D_eff (LDS tables); the face-code sweep and the layer sums take up
+    # to MAX_LABELS labels
     MAX_CLASSES = 64
+    MAX_LABELS = 255
     # sweep through per-voxel face codes when the diffusivities allow it
     # (face_code_tables); False keeps the class-code sweep
     face_codes = True
@@ -839,8 +843,8 @@ class MultiPhaseSolver(ThroughTransportSolver):
         # classes: 0 = the zero-diffusivity ghost class, then one per label
         # present in the image (labels stay separate so that gradients can be
         # reported per label)
-        if len(labels) + 1 > self.MAX_CLASSES:
-            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct labels are supported")
+        if len(labels) > self.MAX_LABELS:
+            raise ValueError(f"at most {self.MAX_LABELS} distinct labels are supported")
         d_cls = np.concatenate([np.zeros(1, dtype=np.float32), d_label]).astype(np.float32)
         cls_of_label = np.arange(1, len(labels) + 1, dtype=np.uint8)
         # face conductance table: hm(a,b) = 2*a*b/(a+b), 0 where a+b == 0
@@ -863,6 +867,9 @@ class MultiPhaseSolver(ThroughTransportSolver):
         # or Z alternates ghost wraps and single half-sweeps on the class codes)
         fused = getattr(self, "_fused_periodic", True)
         faces = face_code_tables(d_cls) if self.face_codes and fused else None
+        if faces is None and len(d_cls) > self.MAX_CLASSES:
+            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct labels are supported unless the "
+                             f"diffusivities take at most {_lib.MPF_MAX_CLASSES} values (0 included)")
         self._faces = None
         if faces is not None:
             cmap, keys, table = faces
@@ -926,6 +933,8 @@ class MultiPhaseSolver(ThroughTransportSolver):
         reference solves under torch.no_grad).  The derivative of the discrete
         fixed point: see include/tauhip.h, tf_grad_classes."""
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
+        if self._n_classes > self.MAX_CLASSES:
+            raise ValueError(f"grad_D_eff supports at most {self.MAX_CLASSES - 1} distinct labels")
         out = torch.empty((B, X, self._n_classes), dtype=torch.float64, device=self.device)
         with torch.cuda.device(self.device):
             st = self._lib.tf_grad_classes(_lib.ptr(self.field), _lib.ptr(self._codes),

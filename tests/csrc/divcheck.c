@@ -1,7 +1,9 @@
-/* Check of the division used by the Solver update (div_by_count in
- * taufactor_amd/csrc/tauhip.hip): for d in {inf, 1..8},
- *   q = s*r;  if d finite: q = fma(fma(-q, d, s), r, q);  if |s| < 2^-100: q = s/d
- * with r = RN(1/d) must equal the IEEE quotient s/d bit for bit.
+/* Check of the division used by the Solver update (the Solver kind of the
+ * register-blocked sweep, taufactor_amd/csrc/tauhip_sweep.h): for d in
+ * {inf, 1..8}, with the table pair (r, d) = (RN(1/d), d) and (0, 0) for inf,
+ *   q = s*r;  e = fma(-q, d, s);  q = fma(e, r, q);  if e is subnormal: q = s/d
+ * must equal the IEEE quotient s/d bit for bit (a zero of either sign counts
+ * as equal: the update c + w (q - c) is the same float for both).
  * argv[1] = "full": all 2^32 inputs; otherwise every mantissa of both signs
  * at a set of exponents (guard boundary, working range, extremes) plus all
  * subnormals.  Prints the number of mismatches; exit status 1 if any. */
@@ -20,17 +22,19 @@ static long long check_range(float df, float r, int finite, uint64_t lo, uint64_
         float s = u2f((uint32_t)i);
         if (isnan(s) || isinf(s)) continue;
         float ref = s / df;
+        const float dt = finite ? df : 0.0f;  /* the table's (0, 0) pair for inf */
         float q = s * r;
-        if (finite) { float e = fmaf(-q, df, s); q = fmaf(e, r, q); }
-        if (fabsf(s) < 0x1p-100f) q = s / df;
-        if (f2u(q) != f2u(ref)) bad++;
+        float e = fmaf(-q, dt, s);
+        q = fmaf(e, r, q);
+        if (e != 0.0f && fabsf(e) < 0x1p-126f) q = s / df; /* subnormal residual */
+        if (f2u(q) != f2u(ref) && !(q == 0.0f && ref == 0.0f)) bad++;
     }
     return bad;
 }
 
 int main(int argc, char** argv) {
     int full = argc > 1 && strcmp(argv[1], "full") == 0;
-    static const int exps[] = {-126, -101, -100, -99, -98, -60, -3, -2, -1, 0, 1, 2, 127};
+    static const int exps[] = {-126, -104, -103, -102, -101, -100, -99, -98, -60, -3, -2, -1, 0, 1, 2, 127};
     long long total = 0;
     for (int d = 0; d <= 8; d++) {
         volatile float one = 1.0f;

@@ -1,12 +1,14 @@
 """The Solver update divides by the neighbour count with a reciprocal product
-and one residual correction instead of an IEEE divide (div_by_count in
-taufactor_amd/csrc/tauhip.hip).  This CPU test compiles tests/csrc/divcheck.c,
-which replays that sequence with C's fmaf and compares it bit for bit with
-the IEEE quotient for every divisor the Solver uses.  The default run covers
-every mantissa at the exponents around the 2^-100 guard, the working range
-and the extremes; TAUHIP_DIVCHECK_FULL=1 runs all 2^32 inputs (about 90 s on
-8 cores; it reports 0 mismatches).  The GPU parity tests cover the compiled
-kernel end to end."""
+and one residual correction instead of an IEEE divide (div_by_count and the
+Solver kind of the sweep in taufactor_amd/csrc/tauhip_sweep.h); a subnormal
+residual sends the quotient to the IEEE division.  This CPU test compiles
+tests/csrc/divcheck.c, which replays that sequence with C's fmaf and compares
+it bit for bit with the IEEE quotient for every divisor the Solver uses (a
+zero of either sign counts as equal: the update is the same float).  The
+default run covers every mantissa at the exponents around the subnormal-
+residual boundary, the working range and the extremes;
+TAUHIP_DIVCHECK_FULL=1 runs all 2^32 inputs (0 mismatches).  The GPU parity
+tests cover the compiled kernel end to end."""
 import os
 import shutil
 import subprocess

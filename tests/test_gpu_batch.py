@@ -74,3 +74,33 @@ def test_batch_iteration_limit_and_reset():
     b.reset()
     b.solve(iter_limit=200)
     assert np.array_equal(first, b.tau) or np.allclose(first, b.tau, equal_nan=True)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_work_queue_equals_full_dispatch(use_graph):
+    """Passes as a work queue over the live tiles (the default) and passes
+    that dispatch every tile give the same iterations and fields; volumes
+    stop at different checks, so later passes run on fewer live tiles."""
+    vols = _volumes(seed=5, n=14)
+    a = tau.BatchSolver(vols)
+    b = tau.BatchSolver(vols)
+    b.use_queue = False
+    for s in (a, b):
+        s.half_sweeps_per_pass = 3
+        s.tile = (8, 2, 16)
+        s.solve(use_graph=use_graph)
+    assert len(set(a.iters.tolist())) > 2
+    assert np.array_equal(a.iters, b.iters) and np.array_equal(a.tau, b.tau)
+    for v in range(len(vols)):
+        assert np.array_equal(a.field(v).cpu().numpy(), b.field(v).cpu().numpy()), v
+
+
+def test_solve_twice_gives_the_same_result():
+    vols = _volumes(seed=6, n=5)
+    b = tau.BatchSolver(vols)
+    b.solve()
+    first = (b.iters.copy(), b.tau.copy(), [b.field(v).cpu().numpy() for v in range(len(vols))])
+    b.solve()
+    assert np.array_equal(first[0], b.iters) and np.array_equal(first[1], b.tau)
+    for v in range(len(vols)):
+        assert np.array_equal(first[2][v], b.field(v).cpu().numpy())

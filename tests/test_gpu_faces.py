@@ -98,9 +98,12 @@ def test_periodic_odd_sizes_keep_the_class_codes():
 
 
 @pytest.mark.parametrize("H", [1, 4])
-def test_face_codes_tiny_and_subnormal_sums(H):
-    """Neighbour sums below 2^-100 take the IEEE division (the rare path)."""
-    Ds = DS[3]
+@pytest.mark.parametrize("ds", [3, 0])
+def test_face_codes_tiny_and_subnormal_sums(H, ds):
+    """Neighbour sums below 2^-100 take the IEEE division (the rare path);
+    with the prefactors of {0, 1, 0.1} many sums are subnormal, where a
+    residual can round to zero and only the guard on s is exact."""
+    Ds = DS[ds]
     img = _labels((12, 10, 14), sorted(Ds), seed=21)
     rng = np.random.default_rng(5)
     vals = np.array([0.0, 2.0 ** -110, -(2.0 ** -112), 3e-40, -1e-42, 7e-45, 2.0 ** -101,
@@ -158,3 +161,26 @@ def test_autotuned_face_code_sweep_at_tuning_size():
     assert autotune.last_timings, "the tuner ran"
     o = OracleMultiPhaseSolver(img, dict(Ds)).run(7)
     assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+
+
+def test_many_labels_with_few_diffusivities():
+    """More than 63 labels (the class-code sweep's limit) when the labels
+    take at most four diffusivities: the face-code sweep serves them, the
+    layer sums read the per-label class codes, and a full solve equals the
+    oracle's; more labels with more diffusivities are refused."""
+    rng = np.random.default_rng(17)
+    labels = np.arange(100)
+    Ds = {int(l): [0.0, 1.0, 0.1][l % 3] for l in labels}
+    img = rng.choice(labels, size=(18, 16, 20))
+    s = tau.MultiPhaseSolver(img, dict(Ds))
+    assert s._faces is not None and s._n_classes == 101
+    s.solve(verbose=False, conv_crit=1e-3)
+    o = OracleMultiPhaseSolver(img, dict(Ds)).solve(conv_crit=1e-3)
+    assert s.iter == o.iter
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+    assert float(s.tau[0]) == float(np.float32(o.tau[0]))
+    with pytest.raises(ValueError):
+        s.grad_D_eff()
+    many = {int(l): 0.1 * (l % 7) for l in labels}
+    with pytest.raises(ValueError):
+        tau.MultiPhaseSolver(img, many)
