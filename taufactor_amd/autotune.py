@@ -23,7 +23,9 @@ import torch
 from . import _lib
 
 MIN_VOXELS = 1 << 23
-TIMED_LAUNCHES = 3
+# even, so that a candidate with an odd H is timed on as many launches of
+# each starting colour
+TIMED_LAUNCHES = 6
 
 # (H values, workgroup shapes, x-tile lengths), in order of preference: the
 # first candidate within TIE of the fastest wins, which keeps the choice
@@ -89,10 +91,13 @@ def tune_solver(s, fixed_h=None, fixed_tile=None):
         state = {"i": 0}
 
         def launch():
+            # launch i starts at half-sweep i*H, so with H odd the starting
+            # colour alternates, as in a solve (the two colours are separate
+            # code paths of the kernel and need not run equally fast)
             i = state["i"]
             state["i"] += 1
             return lib.tf_sweep(_lib.ptr(bufs[i & 1]), _lib.ptr(bufs[1 - (i & 1)]), codes, hm,
-                                n_table, kind, B, X, Y, Z, s._w32, 0, H, t, stream)
+                                n_table, kind, B, X, Y, Z, s._w32, (i * H) & 1, H, t, stream)
 
         ms = _time(launch, TIMED_LAUNCHES)
         if ms is not None:
