@@ -519,6 +519,7 @@ extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int
     a.v = make_vol(X, Y, Z);
     a.ncls = n_classes;
     a.TX = t.tx;
+    a.periodic = periodic;
     a.resolve_only = 1;
     if (int rc = sweep_dispatch(kind, n_half, a, B, t, nullptr)) return rc;
     out->waves = t.waves;

@@ -219,6 +219,7 @@ struct SweepArgs {
     long long fsec;    // field kind: floats between the gx, gy, gz face sections
     int resolve_only;  // set TX and the tile counts, launch nothing
     int periodic;      // y/z neighbours wrap round (Y and Z even)
+    int whole_z;       // one z window per row covers z = 1..Z (Z <= 128, closed)
 };
 
 // compile-time loop: f(std::integral_constant<int, I>{}) for I in [B, E)
@@ -347,6 +348,16 @@ __device__ __forceinline__ float from_next_lane(uint32_t v) {
 // with the unclipped stream start S0 = x0 - (H-1) the active element of a
 // lane depends only on the launch's colour and the step and row parities.
 //
+// A whole-z tile (wz; closed y/z boundaries, Z <= 128) has its window start
+// at z = 1: the 64 lanes hold z = 1..128, so no z halo is recomputed and one
+// wave covers the row.  The z ghosts then lie outside the window (z = 0) or,
+// for Z = 128, past its end (z = 129), and the DPP shifts put 0 there: the
+// ghost's value (closed boundaries keep the y/z ghosts at 0) and, read as a
+// class code or a face-code offset, the zero class and the zero face
+// conductance (the field kind's z- face of z = 1 is the face stored at the
+// ghost, 0 as well).  Used where the strips below would cover Z with more
+// than one wave per row (TZ < Z <= 128).
+//
 // A narrow tile (nar) covers the last strip of z when it is at most TZN wide:
 // each half-wave holds a 64-voxel window of z, half 0 for the rows of y tile
 // 2*ty and half 1 for those of y tile 2*ty + 1 (TY rows further, an even
@@ -382,7 +393,8 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
                                             int colour0, int TX, int z0, int ty, int tx, bool nar,
-                                            long long fsec = 0, int ncls = 0, bool wrap = false) {
+                                            long long fsec = 0, int ncls = 0, bool wrap = false,
+                                            bool wz = false) {
     using Sh = RegShape<H, NW, RPW>;
     // field kind: Cg holds three sections of face conductances in the padded
     // field layout, fsec floats apart: gx, gy, gz, each stored at the voxel
@@ -412,11 +424,11 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     const int y0h[2] = {1 + (nar ? 2 * ty : ty) * TY, 1 + (nar ? 2 * ty + 1 : ty) * TY};
     const int x0 = 1 + tx * TX;
     const int x1 = min(x0 + TX, v.X + 1);
-    const int z1 = min(z0 + (nar ? Sh::TZN : Sh::TZ), v.Z + 1);
+    const int z1 = wz ? v.Z + 1 : min(z0 + (nar ? Sh::TZN : Sh::TZ), v.Z + 1);
     const int half = nar ? lane >> 5 : 0;                 // this lane's half
     const int y0 = half ? y0h[1] : y0h[0];                  // this lane's y tile
     const int ey0 = y0 - H;
-    const int ew = z0 - ML;                                 // odd
+    const int ew = wz ? 1 : z0 - ML;                        // odd
     const int zl = ew + 2 * (nar ? (lane & 31) : lane);     // z of the lane's first voxel
     const int lx1 = min(x1 + H, v.X + 2);
 
@@ -886,7 +898,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
     rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
                                   nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx, nar,
-                                  a.fsec, a.ncls, a.periodic != 0);
+                                  a.fsec, a.ncls, a.periodic != 0, a.whole_z != 0);
 }
 
 // Tile geometry of the kernel k_sweep<H, KIND, NW, RPW> runs: owned rows
@@ -986,7 +998,10 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
         a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
         a.nTyN = 0;
-        {
+        a.whole_z = !a.periodic && a.v.Z > Sh::TZ && a.v.Z <= 128;
+        if (a.whole_z) {
+            a.nTz = 1;
+        } else {
             const ZTiling zt = z_tiling(a.v.Z, Sh::TZ, RegShape<H, NW, RPW>::TZN);
             a.nTz = zt.wide;
             if (zt.narrow) a.nTyN = (a.nTy + 1) / 2;
