@@ -796,8 +796,8 @@ class MultiPhaseSolver(ThroughTransportSolver):
 
     kind = _lib.KIND_MULTIPHASE
     # classes (labels + the zero ghost class) of the class-code sweep and of
-    # grad_D_eff (LDS tables); the face-code sweep and the layer sums take up
-    # to MAX_LABELS labels
+    # grad_D_eff (LDS tables); the face-code sweep, the per-voxel field sweep
+    # (closed boundaries) and the layer sums take up to MAX_LABELS labels
     MAX_CLASSES = 64
     MAX_LABELS = 255
     # sweep through per-voxel face codes when the diffusivities allow it
@@ -867,11 +867,24 @@ class MultiPhaseSolver(ThroughTransportSolver):
         # or Z alternates ghost wraps and single half-sweeps on the class codes)
         fused = getattr(self, "_fused_periodic", True)
         faces = face_code_tables(d_cls) if self.face_codes and fused else None
-        if faces is None and len(d_cls) > self.MAX_CLASSES:
-            raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct labels are supported unless the "
-                             f"diffusivities take at most {_lib.MPF_MAX_CLASSES} values (0 included)")
         self._faces = None
-        if faces is not None:
+        self._field_faces = None
+        if faces is None and len(d_cls) > self.MAX_CLASSES:
+            if not self._zero_yz_ghosts:
+                raise ValueError(f"at most {self.MAX_CLASSES - 1} distinct labels are supported with periodic "
+                                 f"boundaries unless the diffusivities take at most {_lib.MPF_MAX_CLASSES} "
+                                 "values (0 included)")
+            # more classes than the class-code table holds: the per-voxel
+            # field sweep over the face conductances of the padded
+            # diffusivity map d_cls[class] (the same harmonic means, x ghosts
+            # copying the first and last layer, and the same update)
+            self._field_faces = {
+                "kpad": _lib.slack_buffer(shape, torch.float32, self.device),
+                "faces": _lib.slack_buffer((3,) + shape, torch.float32, self.device),
+            }
+        if self._field_faces is not None:
+            self._sweep_codes = None
+        elif faces is not None:
             cmap, keys, table = faces
             self._faces = {
                 "cmap": torch.from_numpy(cmap).to(self.device),
@@ -901,11 +914,21 @@ class MultiPhaseSolver(ThroughTransportSolver):
                                                     _lib.ptr(f["keys"]), _lib.ptr(f["codes"]), B, X, Y, Z,
                                                     self._stream())
             _lib.check(st, "tf_multiphase_face_codes")
+        elif self._field_faces is not None:
+            ff = self._field_faces
+            d = self._d_cls_dev.index_select(0, self._codes.view(-1).to(torch.int32))
+            ff["kpad"].copy_(d.view(ff["kpad"].shape))
+            del d
+            st = self._lib.tf_field_faces(_lib.ptr(ff["kpad"]), _lib.ptr(ff["faces"]), B, X, Y, Z,
+                                          self._stream())
+            _lib.check(st, "tf_field_faces")
         else:
             self._pack_sweep_codes()
         self.vol_x = self._count.cpu().numpy().astype(np.float32) / np.float32(Y * Z)
 
     def _sweep_target(self):
+        if self._field_faces is not None:
+            return _lib.KIND_FIELD, self._field_faces["faces"], None, 0
         if self._faces is None:
             return super()._sweep_target()
         t = self._faces["table"]

@@ -11,7 +11,9 @@ diffusivities fit the face tables (at most four distinct values with 0).
   path and still equals the oracle;
 * diffusivities that do not fit fall back to the class-code sweep, which
   still equals the oracle;
-* a full solve stops at the oracle's iteration with its tau."""
+* a full solve stops at the oracle's iteration with its tau;
+* more than 63 labels with more than four diffusivities sweep as the
+  per-voxel field kind and still equal the oracle."""
 import numpy as np
 import pytest
 import torch
@@ -167,7 +169,7 @@ def test_many_labels_with_few_diffusivities():
     """More than 63 labels (the class-code sweep's limit) when the labels
     take at most four diffusivities: the face-code sweep serves them, the
     layer sums read the per-label class codes, and a full solve equals the
-    oracle's; more labels with more diffusivities are refused."""
+    oracle's; their per-phase gradient is refused (63-label LDS table)."""
     rng = np.random.default_rng(17)
     labels = np.arange(100)
     Ds = {int(l): [0.0, 1.0, 0.1][l % 3] for l in labels}
@@ -181,6 +183,31 @@ def test_many_labels_with_few_diffusivities():
     assert float(s.tau[0]) == float(np.float32(o.tau[0]))
     with pytest.raises(ValueError):
         s.grad_D_eff()
-    many = {int(l): 0.1 * (l % 7) for l in labels}
-    with pytest.raises(ValueError):
-        tau.MultiPhaseSolver(img, many)
+
+
+@pytest.mark.parametrize("H", [1, 4, 6])
+def test_many_labels_with_many_diffusivities(H):
+    """More than 63 labels taking more than four diffusivities: the per-voxel
+    field sweep over the face conductances of the label diffusivities serves
+    them, and its iterates equal the MultiPhase oracle's; a full solve stops
+    at the oracle's iteration with its tau.  With periodic boundaries (no
+    field sweep there) they are refused."""
+    rng = np.random.default_rng(19 + H)
+    labels = np.arange(100)
+    many = {int(l): float(np.float32(0.1 * (l % 7))) for l in labels}
+    img = rng.choice(labels, size=(14, 18, 22))
+    s = tau.MultiPhaseSolver(img, dict(many))
+    assert s._faces is None and s._field_faces is not None and s._n_classes == 101
+    s.half_sweeps_per_pass = H
+    s.solve(iter_limit=2 * H + 9, verbose=False)
+    o = OracleMultiPhaseSolver(img, dict(many)).run(2 * H + 9)
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+    if H == 4:
+        s = tau.MultiPhaseSolver(img, dict(many))
+        s.solve(verbose=False, conv_crit=1e-3)
+        o = OracleMultiPhaseSolver(img, dict(many)).solve(conv_crit=1e-3)
+        assert s.iter == o.iter
+        assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+        assert float(s.tau[0]) == float(np.float32(o.tau[0]))
+        with pytest.raises(ValueError):
+            tau.PeriodicMultiPhaseSolver(rng.choice(labels, size=(6, 8, 10)), dict(many))
