@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define TF_ABI_VERSION 11
+#define TF_ABI_VERSION 12
 
 #define TF_KIND_SOLVER 0
 #define TF_KIND_MULTIPHASE 1
@@ -453,6 +453,13 @@ int tf_batch_check(const tf_volume* vols, int nvol, const double* flux_sum,
  * (taufactor/metrics/connectivity.py:138-213, find_spanning_labels :83-115)
  * as called from compute_metrics (taufactor/taufactor.py:318-327).        */
 int tf_spans_x(const uint8_t* mask, int X, int Y, int Z);
+
+/* Device-to-device copy of `bytes` bytes (a multiple of 16; both pointers
+ * 16-byte aligned) by a streaming kernel: nontemporal 16-byte loads and
+ * stores, four per lane in flight, one pass over the buffer.  No reference counterpart: it
+ * calibrates the HBM traffic counters and gives the device's copy rate
+ * against which the sweep's measured traffic is read.                    */
+int tf_hbm_copy(const void* src, void* dst, long long bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtauhip.so")
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 KIND_SOLVER = 0
 KIND_MULTIPHASE = 1
@@ -85,6 +85,7 @@ SIGNATURES = {
                                       _c_int, ctypes.POINTER(Tile), _vp]),
     "tf_batch_layer_sums": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp, _vp, _vp]),
     "tf_batch_check": (_c_int, [_vp, _c_int, _vp, _vp, _c_int, _c_int, _c_int, ctypes.c_double, _vp]),
+    "tf_hbm_copy": (_c_int, [_vp, _vp, ctypes.c_longlong, _vp]),
 }
 
 # numpy mirrors of the batch structs (layout checked in tests/test_abi.py)

@@ -1501,3 +1501,46 @@ extern "C" int tf_spans_x(const uint8_t* mask, int X, int Y, int Z) {
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// streaming copy (HBM calibration)
+// ---------------------------------------------------------------------------
+// Each lane moves four 16-byte vectors, 256 lanes apart, so a workgroup
+// covers 16 KiB with every load issued before the first store; one pass,
+// no grid-stride loop (the grid is n / 1024 workgroups).  Nontemporal loads
+// and stores: the data passes through once (5.97 TB/s for 2 GiB on MI355X,
+// 5.67 with the default cache policy).
+constexpr int kCopyThreads = 256;
+constexpr int kCopyPerLane = 4;
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(kCopyThreads) void k_hbm_copy(const u32x4_t* __restrict__ s,
+                                                           u32x4_t* __restrict__ d, long long n16) {
+    const long long base = (long long)blockIdx.x * (kCopyThreads * kCopyPerLane) + threadIdx.x;
+    u32x4_t v[kCopyPerLane];
+#pragma unroll
+    for (int k = 0; k < kCopyPerLane; ++k) {
+        const long long i = base + (long long)k * kCopyThreads;
+        if (i < n16) v[k] = __builtin_nontemporal_load(s + i);
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyPerLane; ++k) {
+        const long long i = base + (long long)k * kCopyThreads;
+        if (i < n16) __builtin_nontemporal_store(v[k], d + i);
+    }
+}
+
+extern "C" int tf_hbm_copy(const void* src, void* dst, long long bytes, void* stream) {
+    if (!src || !dst) return fail(-1, "null pointer argument");
+    if (bytes < 0 || (bytes & 15)) return fail(-1, "tf_hbm_copy needs a multiple of 16 bytes (got %lld)", bytes);
+    if (((uintptr_t)src | (uintptr_t)dst) & 15) return fail(-1, "tf_hbm_copy needs 16-byte aligned pointers");
+    if (bytes == 0) return 0;
+    const long long n16 = bytes / 16;
+    const long long per = (long long)kCopyThreads * kCopyPerLane;
+    const long long grid = (n16 + per - 1) / per;
+    if (grid > 0x7fffffffLL) return fail(-1, "tf_hbm_copy: %lld bytes is too large", bytes);
+    hipLaunchKernelGGL(k_hbm_copy, dim3((unsigned)grid), dim3(kCopyThreads), 0, (hipStream_t)stream,
+                       static_cast<const u32x4_t*>(src), static_cast<u32x4_t*>(dst), n16);
+    TF_LAUNCH_CHECK("k_hbm_copy");
+    return 0;
+}

@@ -380,3 +380,20 @@ def test_tiny_and_subnormal_sums_take_the_ieee_division(H):
     got, want = _field(s), o.field.numpy()
     assert np.array_equal(got, want)
     assert np.count_nonzero((np.abs(want) < 2.0 ** -126) & (want != 0)) > 0  # subnormals present
+
+
+def test_hbm_copy_kernel():
+    """tf_hbm_copy (the bandwidth calibration kernel of bench.py) copies every
+    byte, also when the size is not a multiple of a workgroup's 16 KiB, writes
+    nothing past the end, and refuses sizes that are not a multiple of 16."""
+    from taufactor_amd import _lib
+
+    lib = _lib.load()
+    stream = _lib.stream_handle(torch.device("cuda", 0))
+    for n in (4, 4096, 4100, 3 * 2 ** 20 + 12):
+        src = torch.randint(-2 ** 31, 2 ** 31 - 1, (n,), dtype=torch.int32, device="cuda")
+        dst = torch.zeros(n + 64, dtype=torch.int32, device="cuda")
+        assert lib.tf_hbm_copy(_lib.ptr(src), _lib.ptr(dst), 4 * n, stream) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(dst[:n], src) and not torch.any(dst[n:])
+    assert lib.tf_hbm_copy(_lib.ptr(src), _lib.ptr(dst), 20, stream) != 0
