@@ -65,20 +65,24 @@ if ROOT not in sys.path:
 METRIC = "GVoxel-updates/s + % HBM roofline; solves/s batched vs plain @ D_eff rel err≤1e-3"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"code": 9, "field": 12}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_solver_512_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweep_traffic.json")
+# the compilation unit of each workload's sweep kernel (its traffic entry is
+# keyed to the hash of that unit, the shared sweep header and the flags)
+SWEEP_UNITS = {"C2": "sweep_solver.hip", "C4": "sweep_mpfaces.hip", "C5": "sweep_field.hip"}
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def sweep_source_sha256() -> str:
-    """Hash of what the Solver sweep kernel is built from: its sources and the
-    compiler flags.  Keys the committed rocprof traffic summary to a build."""
+def sweep_source_sha256(unit: str = "sweep_solver.hip") -> str:
+    """Hash of what a sweep kernel is built from: the shared sweep header,
+    the kind's compilation unit and the compiler flags.  Keys the committed
+    rocprof traffic summary to a build."""
     from taufactor_amd.csrc import build
 
     h = hashlib.sha256()
-    for name in ("tauhip_sweep.h", "sweep_solver.hip"):
+    for name in ("tauhip_sweep.h", unit):
         with open(os.path.join(build.HERE, name), "rb") as f:
             h.update(f.read())
     h.update(" ".join(build.HIPCC_FLAGS).encode())
@@ -94,26 +98,28 @@ def roofline(kind: str, nvox: int, H: int, mean_launch_s: float, kernel: str) ->
             "mean_launch_ms": round(mean_launch_s * 1e3, 4), "traffic": None}
 
 
-def attach_traffic(rl: dict, H: int, tile, mean_launch_s: float) -> None:
+def attach_traffic(rl: dict, workload: str, H: int, tile, mean_launch_s: float) -> None:
     """Measured HBM bytes per launch from the committed summary of this
-    kernel build, when there is one for this depth and tile."""
+    kernel build, when there is one for this workload, depth and tile."""
     if not os.path.exists(TRAFFIC_FILE):
         rl["traffic_note"] = "no traffic summary committed"
         return
     with open(TRAFFIC_FILE) as f:
         entries = json.load(f)
-    src = sweep_source_sha256()
-    for t in entries if isinstance(entries, list) else [entries]:
-        if t.get("half_sweeps_per_pass") == H and t.get("tile") == list(tile) and t.get("source_sha256") == src:
+    src = sweep_source_sha256(SWEEP_UNITS[workload])
+    for t in entries:
+        if (t.get("workload") == workload and t.get("half_sweeps_per_pass") == H and t.get("tile") == list(tile)
+                and t.get("source_sha256") == src):
             b = t["hbm_bytes_per_launch"]
             rl["traffic"] = b
             rl["dram_GBps"] = round(b / mean_launch_s / 1e9, 1)
             rl["dram_frac"] = round(b / mean_launch_s / 1e9 / HBM_PEAK_GBS, 4)
             rl["traffic_source"] = (f"rocprofv3 FETCH_SIZE x {t['fetch_factor']} + WRITE_SIZE x "
-                                    f"{t['write_factor']} per launch of {t['kernel']}, "
-                                    "profiles/k_sweep_solver_512_traffic.md, same kernel-source hash")
+                                    f"{t['write_factor']} per launch of {t['kernel']}, profiles/{t['profile']}, "
+                                    "same kernel-source hash")
             return
-    rl["traffic_note"] = f"no traffic summary for this kernel build (source sha256 {src[:16]}), H={H}, tile {list(tile)}"
+    rl["traffic_note"] = (f"no traffic summary for this kernel build (source sha256 {src[:16]}), H={H}, "
+                          f"tile {list(tile)}")
 
 
 def kernel_mean_s(events) -> float:
@@ -182,7 +188,7 @@ def run_solver512(args, dev):
     nvox = img.size
     gvus = nvox * sum(iters) / 2 / elapsed / 1e9
     rl = roofline("code", nvox, H, mean_s, f"k_sweep<{H}, Solver, {rtile[0]}, {rtile[1]}>")
-    attach_traffic(rl, H, rtile, mean_s)
+    attach_traffic(rl, "C2", H, rtile, mean_s)
     log(f"{args.steps} solves, iterations {iters}, {elapsed:.3f} s -> {gvus:.1f} GVoxel-updates/s; "
         f"sweep kernel {mean_s * 1e3:.4f} ms -> effective frac {rl['frac']}, dram frac {rl.get('dram_frac')}")
     out = {
@@ -263,6 +269,13 @@ def run_batch256(dev) -> dict:
     return out
 
 
+def _with_launch(s, launch):
+    """s with the launch configuration (H, tile) set, if one is given."""
+    if launch:
+        s.half_sweeps_per_pass, s.tile = launch
+    return s
+
+
 def _grad_timing(make, forward_crit: float, grad):
     """Forward at forward_crit, then one gradient; (solver, forward s, backward s)."""
     import torch
@@ -285,8 +298,9 @@ def _grad_timing(make, forward_crit: float, grad):
     return s, g, t_fwd, t_bwd
 
 
-def run_multiphase320(dev) -> dict:
-    """C4: MultiPhaseSolver 320^3 forward; per-phase gradient."""
+def run_multiphase320(dev, launch=None) -> dict:
+    """C4: MultiPhaseSolver 320^3 forward; per-phase gradient.  launch: (H,
+    tile) of the forward instead of the autotuner's choice."""
     import torch
 
     import taufactor_amd as tau
@@ -296,7 +310,7 @@ def run_multiphase320(dev) -> dict:
     img = workloads.three_phase(320, seed=0)
     log(f"C4: volume {img.shape} ({time.perf_counter() - t0:.1f} s)")
     Ds = {0: 0.0, 1: 1.0, 2: 0.1}
-    s = tau.MultiPhaseSolver(img, dict(Ds), device=dev)
+    s = _with_launch(tau.MultiPhaseSolver(img, dict(Ds), device=dev), launch)
     s.solve(verbose=False)
     H, tile, n_tiles = s.resolved_launch()
     s._launch_events = []
@@ -312,10 +326,14 @@ def run_multiphase320(dev) -> dict:
     gvus = nvox * s.iter / 2 / t_fwd / 1e9
     fwd = {"iterations": s.iter, "tau": float(s.tau[0]), "D_eff": float(s.D_eff[0]), "ms": round(t_fwd * 1e3, 3),
            "GVoxel_updates_per_s": round(gvus, 3)}
-    rl = roofline("code", nvox, H, mean_s, f"k_sweep<{H}, MultiPhase, {tile[0]}, {tile[1]}>")
+    faces = s._faces is not None
+    rl = roofline("code", nvox, H, mean_s,
+                  f"k_sweep<{H}, {'MultiPhaseFaces' if faces else 'MultiPhase'}, {tile[0]}, {tile[1]}>")
+    if faces:
+        attach_traffic(rl, "C4", H, tile, mean_s)
     del s
-    s, g, t_f, t_b = _grad_timing(lambda: tau.MultiPhaseSolver(img, dict(Ds), device=dev), 1e-3,
-                                  lambda x: x.grad_D_eff())
+    s, g, t_f, t_b = _grad_timing(lambda: _with_launch(tau.MultiPhaseSolver(img, dict(Ds), device=dev), launch),
+                                  1e-3, lambda x: x.grad_D_eff())
     log(f"C4: forward {t_fwd * 1e3:.1f} ms ({fwd['iterations']} iterations, {gvus:.1f} GVU/s, kernel "
         f"{mean_s * 1e3:.4f} ms, frac {rl['frac']}); conv 1e-3 forward {t_f * 1e3:.1f} ms, "
         f"gradient {t_b * 1e3:.2f} ms")
@@ -334,7 +352,7 @@ def run_multiphase320(dev) -> dict:
     return out
 
 
-def run_field128x64(dev) -> dict:
+def run_field128x64(dev, launch=None) -> dict:
     """C5: 64 volumes of 128^3 with per-voxel k, batched forward; per-voxel gradient."""
     import torch
 
@@ -345,7 +363,7 @@ def run_field128x64(dev) -> dict:
     k = workloads.sigmoid_k(64, 128, seed=0)
     log(f"C5: k {k.shape} ({time.perf_counter() - t0:.1f} s)")
     kt = torch.from_numpy(k).to(dev)
-    s = tau.FieldSolver(kt, device=dev)
+    s = _with_launch(tau.FieldSolver(kt, device=dev), launch)
     s.solve(verbose=False)
     H, tile, n_tiles = s.resolved_launch()
     s._launch_events = []
@@ -362,8 +380,10 @@ def run_field128x64(dev) -> dict:
     fwd = {"iterations": s.iter, "D_eff_mean": float(s.D_eff.mean()), "ms": round(t_fwd * 1e3, 3),
            "GVoxel_updates_per_s": round(gvus, 3)}
     rl = roofline("field", nvox, H, mean_s, f"k_sweep<{H}, Field, {tile[0]}, {tile[1]}>")
+    attach_traffic(rl, "C5", H, tile, mean_s)
     del s
-    s, g, t_f, t_b = _grad_timing(lambda: tau.FieldSolver(kt, device=dev), 1e-4, lambda x: x.grad_D_eff())
+    s, g, t_f, t_b = _grad_timing(lambda: _with_launch(tau.FieldSolver(kt, device=dev), launch), 1e-4,
+                                  lambda x: x.grad_D_eff())
     log(f"C5: forward {t_fwd * 1e3:.1f} ms ({fwd['iterations']} iterations, {gvus:.1f} GVU/s, kernel "
         f"{mean_s * 1e3:.4f} ms, frac {rl['frac']}); conv 1e-4 forward {t_f * 1e3:.1f} ms, "
         f"gradient {t_b * 1e3:.2f} ms")
@@ -547,8 +567,13 @@ def main(argv=None) -> int:
     dev = torch.device("cuda", 0)
     single = {"batch256": run_batch256, "multiphase320": run_multiphase320, "field128x64": run_field128x64}
     if args.config in single:
+        kw = {}
+        if args.config != "batch256" and args.half_sweeps and args.tile:
+            # a fixed launch configuration (profiling runs: the autotuner's
+            # timings under counter collection need not pick the default's)
+            kw["launch"] = (args.half_sweeps, tuple(int(v) for v in args.tile.split(",")))
         with contextlib.redirect_stdout(sys.stderr):
-            r = single[args.config](dev)
+            r = single[args.config](dev, **kw)
         r.pop("_iters", None)
         print(json.dumps({"config": args.config, **r}), flush=True)
         return 0
@@ -565,28 +590,45 @@ def main(argv=None) -> int:
 
 
 def bench_copy(args, dev) -> int:
-    """Copy of a --size^3 float32 field on the device, timed with HIP events."""
+    """Copy of a --size^3 float32 field on the device, timed with HIP events
+    on the launch stream: the library's streaming copy kernel (tf_hbm_copy,
+    the device's copy rate and the traffic-counter calibration) and torch's
+    copy (the runtime's copy kernel)."""
     import torch
 
+    from taufactor_amd import _lib
+
+    lib = _lib.load()
     n = args.size ** 3
     src = torch.rand(n, device=dev)
     dst = torch.empty_like(src)
-    for _ in range(max(args.warmup, 1)):
-        dst.copy_(src)
+    stream = _lib.stream_handle(dev)
     reps = 20 * args.steps
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize(dev)
-    a.record()
-    for _ in range(reps):
-        dst.copy_(src)
-    b.record()
-    torch.cuda.synchronize(dev)
-    ms = a.elapsed_time(b) / reps
-    assert torch.equal(dst, src)
-    gbs = 2 * 4 * n / (ms / 1e3) / 1e9
-    log(f"copy {4 * n / 1e6:.0f} MB: {ms:.4f} ms -> {gbs:.0f} GB/s (read + write)")
-    print(json.dumps({"config": "hbm_copy", "GBps": round(gbs, 1), "ms": round(ms, 5),
-                      "bytes_read": 4 * n, "bytes_written": 4 * n}), flush=True)
+
+    def timed(copy):
+        for _ in range(max(args.warmup, 1)):
+            copy()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize(dev)
+        a.record()
+        for _ in range(reps):
+            copy()
+        b.record()
+        torch.cuda.synchronize(dev)
+        return a.elapsed_time(b) / reps
+
+    def kernel_copy():
+        _lib.check(lib.tf_hbm_copy(_lib.ptr(src), _lib.ptr(dst), 4 * n, stream), "tf_hbm_copy")
+
+    out = {"config": "hbm_copy", "bytes_read": 4 * n, "bytes_written": 4 * n}
+    for name, copy in (("tf_hbm_copy", kernel_copy), ("torch_copy", lambda: dst.copy_(src))):
+        dst.zero_()
+        ms = timed(copy)
+        assert torch.equal(dst, src)
+        gbs = 2 * 4 * n / (ms / 1e3) / 1e9
+        log(f"{name}: copy {4 * n / 1e6:.0f} MB: {ms:.4f} ms -> {gbs:.0f} GB/s (read + write)")
+        out[name] = {"ms": round(ms, 5), "GBps": round(gbs, 1), "frac_of_8TBps": round(gbs / HBM_PEAK_GBS, 4)}
+    print(json.dumps(out), flush=True)
     return 0
 
 
