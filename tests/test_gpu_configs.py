@@ -84,6 +84,15 @@ def test_c4_crop48_forward_and_phase_gradient():
     for lbl in (1, 2):
         fd = g["grad_fd"][str(lbl)]
         assert abs(grad[lbl][0] - fd) <= 1e-3 * abs(fd), (lbl, grad[lbl][0], fd)
+    # the criteria the benchmark times the backward after (1e-3) and the
+    # solvers' default (1e-2): the gradient of the field the forward stops on
+    for crit, bound in ((1e-3, 2e-2), (1e-2, 5e-2)):
+        u = tau.MultiPhaseSolver(lab, dict(Ds))
+        u.solve(verbose=False, conv_crit=crit)
+        gu = u.grad_D_eff()
+        err = max(abs(gu[lbl][0] - g["grad_fd"][str(lbl)]) / abs(g["grad_fd"][str(lbl)]) for lbl in (1, 2))
+        print(f"C4 48^3 crop, conv_crit {crit:g}: {u.iter} iterations, per-phase gradient relative error {err:.2e}")
+        assert err <= bound
 
 
 def test_c5_crop24_voxel_gradient():
