@@ -1254,11 +1254,14 @@ extern "C" int tf_sweep_tile_count(int X, int Y, int Z, int n_half, const tf_til
     if (n_half < 1 || n_half > TF_MAX_FUSED_HALF_SWEEPS) return fail(-1, "n_half=%d out of range", n_half);
     tf_tile t;
     if (int rc = resolve_tile(n_half, tile, &t)) return rc;
-    // the tiles k_sweep_batch decodes: wide tiles, then a narrow strip
+    // the tiles k_sweep_batch decodes: wide tiles, then a narrow strip (one
+    // wide tile per row tile for whole-z windows)
     if ((long long)(t.tx + 2 * n_half) * (Y + 2) * (Z + 2) * 4 >= (1LL << 31) - 64)
         return fail(-1, "x tile %d of %dx%d planes exceeds the sweep's 2 GiB tile span", t.tx, Y, Z);
     const long long ty = (long long)t.waves * t.rows_per_wave - 2 * n_half;
-    const ZTiling zt = z_tiling(Z, reg_tile_tz(n_half), reg_tile_tzn(n_half));
+    const ZTiling zt = whole_z_window(Z, reg_tile_tz(n_half), false)
+                           ? ZTiling{1, false}
+                           : z_tiling(Z, reg_tile_tz(n_half), reg_tile_tzn(n_half));
     const long long nty = (Y + ty - 1) / ty;
     long long n = ((long long)zt.wide * nty + (zt.narrow ? (nty + 1) / 2 : 0)) * ((X + t.tx - 1) / t.tx);
     if (n >= (1LL << 28)) return fail(-1, "too many tiles");

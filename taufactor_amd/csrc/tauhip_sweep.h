@@ -290,6 +290,11 @@ __host__ __device__ inline ZTiling z_tiling(int Z, int TZ, int TZN) {
     const int w = Z / TZ, rest = Z - w * TZ;
     return {w + (rest > TZN ? 1 : 0), rest > 0 && rest <= TZN};
 }
+// whole-z windows (rsweep_tile's wz): closed y/z boundaries and a row that
+// one window of 128 z covers while the strips would need more than one
+__host__ __device__ inline bool whole_z_window(int Z, int TZ, bool periodic) {
+    return !periodic && Z > TZ && Z <= 128;
+}
 // owned z of a wide and of a narrow tile at depth H (RegShape::TZ, ::TZN)
 __host__ __device__ constexpr int reg_tile_tz(int H) { return ((128 - (H + (H & 1)) - H) / 2) * 2; }
 __host__ __device__ constexpr int reg_tile_tzn(int H) { return ((64 - (H + (H & 1)) - H) / 2) * 2; }
@@ -1003,7 +1008,7 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
         a.nTz = (a.v.Z + Sh::TZ - 1) / Sh::TZ;
         a.nTyN = 0;
-        a.whole_z = !a.periodic && a.v.Z > Sh::TZ && a.v.Z <= 128;
+        a.whole_z = whole_z_window(a.v.Z, Sh::TZ, a.periodic != 0);
         if (a.whole_z) {
             a.nTz = 1;
         } else {
@@ -1111,8 +1116,10 @@ __device__ __forceinline__ void batch_tile(const BatchArgs& a, int logical) {
     const tf_volume vol = a.vols[vi];
     const Vol v = make_vol(vol.X, vol.Y, vol.Z);
     // this volume's tiles, numbered as in k_sweep: wide tiles (tz fastest,
-    // then ty, then tx), then the narrow strip (ty, then tx)
-    const ZTiling zt = z_tiling(vol.Z, Sh::TZ, Sh::TZN);
+    // then ty, then tx), then the narrow strip (ty, then tx); a volume of
+    // whole-z windows has one wide tile per (ty, tx)
+    const bool wz = whole_z_window(vol.Z, Sh::TZ, false);
+    const ZTiling zt = wz ? ZTiling{1, false} : z_tiling(vol.Z, Sh::TZ, Sh::TZN);
     const int nTy = (vol.Y + Sh::TY - 1) / Sh::TY;
     const int nTx = (vol.X + a.TX - 1) / a.TX;
     int t = logical - vol.tile_first;
@@ -1134,7 +1141,7 @@ __device__ __forceinline__ void batch_tile(const BatchArgs& a, int logical) {
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
     rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
-                                            vol.omega, a.colour0, a.TX, z0, ty, tx, nar);
+                                            vol.omega, a.colour0, a.TX, z0, ty, tx, nar, 0, 0, false, wz);
 }
 
 template <int H, int NW, int RPW>

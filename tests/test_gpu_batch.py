@@ -104,3 +104,29 @@ def test_solve_twice_gives_the_same_result():
     assert np.array_equal(first[0], b.iters) and np.array_equal(first[1], b.tau)
     for v in range(len(vols)):
         assert np.array_equal(first[2][v], b.field(v).cpu().numpy())
+
+
+@pytest.mark.parametrize("H", [1, 4, 6])
+def test_batch_whole_z_windows(H):
+    """Volumes whose rows one z window covers (TZ < Z <= 128: one wide tile
+    per row tile from z = 1) next to ones with z strips: the batch equals the
+    oracle's iterates and the solo solves stop where the batch does."""
+    from oracle.restate import OracleSolver
+
+    rng = np.random.default_rng(40 + H)
+    shapes = [(9, 14, 128), (7, 30, 125), (6, 11, 121), (8, 12, 60), (5, 9, 130)]
+    imgs = [(rng.random(sh) < 0.6).astype(np.uint8) for sh in shapes]
+    b = tau.BatchSolver(imgs)
+    b.half_sweeps_per_pass = H
+    b.solve(iter_limit=100)  # no volume can stop at its first check
+    for v, img in enumerate(imgs):
+        assert b.iters[v] == 100
+        o = OracleSolver(img).run(100)
+        assert np.array_equal(b.field(v).cpu().numpy(), o.field.numpy()[0]), (v, shapes[v])
+    b = tau.BatchSolver(imgs)
+    b.half_sweeps_per_pass = H
+    b.solve()
+    for v, img in enumerate(imgs):
+        s = _solo(img)
+        assert b.iters[v] == s.iter and b.tau[v] == s.tau[0], v
+        assert np.array_equal(b.field(v).cpu().numpy(), s.field[0].cpu().numpy()), v

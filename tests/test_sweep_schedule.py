@@ -4,7 +4,8 @@ against the oracle.
 The restatement follows rsweep_tile (taufactor_amd/csrc/tauhip_sweep.h) step
 by step: tiles of NW waves x RPW consecutive rows (TY = NW*RPW - 2H owned) by a
 128-voxel z window (TZ owned) or, for the last strip of z when it fits, a
-narrow tile whose two half-waves hold 64-voxel windows of two y tiles; each
+narrow tile whose two half-waves hold 64-voxel windows of two y tiles, or, for
+TZ < Z <= 128 with closed boundaries, one whole-z window from z = 1; each
 lane keeps its z-pair of every row in a ring of NR planes; half-sweep h of step
 s updates plane s-h; z neighbours across lanes come from the adjacent lane
 (the DPP wave shifts; 0 past the ends), the y neighbours of a wave's first and
@@ -43,9 +44,12 @@ def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None, wrap=False):
     D = NR - H - 2
     nTy, nTx = -(-Y // TY), -(-X // TX)
     wide, rest = Z // TZ, Z % TZ
+    wz = not wrap and TZ < Z <= 128
     tiles = [(1 + tz * TZ, ty, False) for tz in range(wide + (rest > TZN)) for ty in range(nTy)]
     if 0 < rest <= TZN:
         tiles += [(1 + wide * TZ, ty, True) for ty in range(-(-nTy // 2))]
+    if wz:
+        tiles = [(1, ty, False) for ty in range(nTy)]
     d_of = np.array([np.inf] + list(range(1, 9)), np.float32)
     lanes = np.arange(L)
     for tx in range(nTx):
@@ -55,6 +59,8 @@ def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None, wrap=False):
             x0 = 1 + tx * TX
             z1, x1 = min(z0 + (TZN if nar else TZ), Z + 1), min(x0 + TX, X + 1)
             ew = z0 - ML
+            if wz:  # the window starts at z = 1; the DPP zero fill stands in for the ghosts
+                z1, ew = Z + 1, 1
             zl = ew + 2 * (lanes & 31 if nar else lanes)
             lx1 = min(x1 + H, X + 2)
             F = np.zeros((NR, NW, RPW, L, 2), np.float32)
@@ -161,6 +167,10 @@ def _binary(shape, seed):
     ((6, 9, 11), 3, 8, 2, 4),     # several x tiles, first stream start below plane 1
     ((4, 20, 140), 4, 4, 4, 2),   # one wide tile + narrow strip, four-row waves
     ((4, 30, 250), 6, 12, 2, 4),  # two wide tiles + narrow strip, odd count of narrow y tiles
+    ((3, 11, 128), 4, 8, 2, 2),   # whole-z window (TZ = 120 < Z = 128)
+    ((3, 9, 125), 1, 4, 4, 3),    # whole-z window, odd Z
+    ((2, 13, 121), 6, 8, 2, 2),   # whole-z window (TZ = 116)
+    ((2, 7, 120), 4, 8, 2, 2),    # Z = TZ: one wide tile, not whole-z
 ])
 def test_schedule_equals_reference_iterations(shape, H, NW, RPW, TX):
     img = _binary(shape, sum(shape) + H)
