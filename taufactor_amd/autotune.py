@@ -41,6 +41,23 @@ _cache: dict = {}
 last_timings: list = []
 
 
+def small_volume_default(voxels: int):
+    """(H, tile) for a volume the tuner skips (fewer than MIN_VOXELS voxels):
+    smaller workgroups put more tiles on the 256 CUs, where a few large
+    tiles would leave most of them idle.  Measured on MI355X (whole solves,
+    Solver / MultiPhase / per-voxel field): below 1.5 M voxels 5 half-sweeps
+    with 8 x 2 workgroups (100^3: 2.63 -> 1.72 ms Solver, 3.36 -> 2.23
+    MultiPhase, 1.42 -> 1.23 field); up to 3 M voxels 4 with 8 x 2 (128^3:
+    2.61 -> 1.92, 3.46 -> 2.56, 1.48 -> 1.50); above that the library's
+    default (4 half-sweeps, 16 x 2) was as fast as either.  (None, None)
+    leaves the library's default."""
+    if voxels < 3 * (1 << 19):
+        return (5, (8, 2, 0))
+    if voxels < 3 * (1 << 20):
+        return (4, (8, 2, 0))
+    return (None, None)
+
+
 def _candidates(space, fixed_h, fixed_tile):
     hs, shapes, txs = space
     if fixed_h is not None:

@@ -198,13 +198,18 @@ class SORSolver(ABC):
     def _launch_config(self):
         """(H, tile) of the next launches: explicit settings win; otherwise the
         autotuner's choice for volumes of at least autotune.MIN_VOXELS voxels
-        (made once, before the first iteration), else H = 4 and the library's
-        default tile."""
+        (made once, before the first iteration), for smaller ones with
+        neither set autotune.small_volume_default, else H = 4 and the
+        library's default tile."""
         H, tile = self.half_sweeps_per_pass, self.tile
         if H is not None and tile is not None:
             return int(H), tuple(tile)
+        voxels = self.batch_size * self.Nx * self.Ny * self.Nz
+        big = voxels >= autotune.MIN_VOXELS
+        if H is None and tile is None and self.autotune and self._tunable and not big:
+            sH, stile = autotune.small_volume_default(voxels)
+            return (sH or DEFAULT_HALF_SWEEPS_PER_PASS), stile
         if self._auto_cfg is None:
-            big = self.batch_size * self.Nx * self.Ny * self.Nz >= autotune.MIN_VOXELS
             if self.autotune and self._tunable and big and self.iter == 0:
                 self._auto_cfg = autotune.tune_solver(self, H, tile)
             else:

@@ -61,3 +61,21 @@ def test_cpu_device_is_refused():
 def test_missing_gpu_is_an_error_not_a_fallback():
     with pytest.raises(RuntimeError, match="needs a HIP GPU"):
         tau.Solver(np.ones((4, 4, 4)))
+
+
+def test_small_volume_default_launch():
+    """Volumes the tuner skips get the measured small-volume configurations:
+    5 half-sweeps with 8 x 2 workgroups below 1.5 M voxels, 4 with 8 x 2
+    below 3 M, then the library's default; every shape they name is one the
+    sweep compiles for every kind (8 x 2 at every depth) and leaves rows to
+    own (8 x 2 rows > 2 H)."""
+    from taufactor_amd import autotune
+
+    assert autotune.small_volume_default(100 ** 3) == (5, (8, 2, 0))
+    assert autotune.small_volume_default(128 ** 3) == (4, (8, 2, 0))
+    assert autotune.small_volume_default(160 ** 3) == (None, None)
+    assert 160 ** 3 < autotune.MIN_VOXELS
+    for v in (1, 10 ** 6, 2 * 10 ** 6, 5 * 10 ** 6):
+        H, tile = autotune.small_volume_default(v)
+        if tile is not None:
+            assert tile[:2] == (8, 2) and 8 * 2 > 2 * H
