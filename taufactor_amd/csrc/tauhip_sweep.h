@@ -560,28 +560,31 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
             }
         }
     };
-    // store k writes the lanes of half k: one descriptor per row, plane and
-    // half, based at z = 0 of that row with record count 4*z1.  The stores
-    // are nontemporal (cache policy nt): the next read of the plane is the
-    // next launch's, so the written lines need not stay in L2, where the
-    // loaded planes' halo lines are reused by neighbouring tiles (512^3
-    // Solver at H = 6: 0.3854 -> 0.3813 ms per launch, H = 4: 0.3476 -> 0.3375)
+    // stores: one descriptor per plane step (based at the plane, record
+    // count the plane's bytes) and one 8-byte store per row; a lane writes
+    // its pair of row j when it owns the pair's first voxel (its half's row j
+    // is owned and z0 <= zl < z1), else its offset lies past the plane.  With
+    // Z odd the last pair's second voxel is the z = Z + 1 ghost, written too:
+    // its register holds what the ghost holds (+0 with closed boundaries,
+    // code 0 keeping it; a periodic volume with Z odd re-wraps its ghosts
+    // before every half-sweep).  The stores are nontemporal (cache policy
+    // nt): the next read of the plane is the next launch's, so the written
+    // lines need not stay in L2, where the loaded planes' halo lines are
+    // reused by neighbouring tiles (512^3 Solver at H = 6: 0.3854 -> 0.3813
+    // ms per launch, H = 4: 0.3476 -> 0.3375)
     constexpr int kStoreNT = 2;
-    const int zs0 = (zl >= z0 && half == 0) ? zl * 4 : kOff;
-    const int zs1 = (zl >= z0 && half == 1) ? zl * 4 : kOff;
+    int so[RPW];
+#pragma unroll
+    for (int j = 0; j < RPW; ++j) {
+        const bool own = (half ? own_row[1][j] : own_row[0][j]) && zl >= z0 && zl < z1;
+        so[j] = own ? ((half ? ys[1][j] : ys[0][j]) * v.sY + zl) * 4 : kOff;
+    }
     auto store_plane = [&](int q, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         const bool qok = q >= x0 && q < x1;
-        const int qs = qok ? q : 0;
+        const rsrc_t r = make_rsrc(D + (size_t)(qok ? q : 0) * v.sX, qok ? (uint32_t)(4 * v.sX) : 0u);
 #pragma unroll
-        for (int j = 0; j < RPW; ++j) {
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const rsrc_t r = make_rsrc(D + (size_t)qs * v.sX + (size_t)ys[k][j] * v.sY,
-                                           (qok && own_row[k][j]) ? (uint32_t)(4 * z1) : 0u);
-                __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, k ? zs1 : zs0, 0, kStoreNT);
-            }
-        }
+        for (int j = 0; j < RPW; ++j) __builtin_amdgcn_raw_buffer_store_b64(F[sl][j], r, so[j], 0, kStoreNT);
     };
 
     // ---- per-tile bounds of each half-sweep (planes only: rows and z are

@@ -148,8 +148,11 @@ def _tile_sweep(f0, codes, H, NW, RPW, TX, colour0, w, bug=None, wrap=False):
                                 y = rows(wv, j)
                                 own = (y >= y0) & (y < np.minimum(y0 + TY, Y + 1))
                                 for e in range(2):
+                                    # a lane writes its whole pair when it owns
+                                    # the first voxel (with Z odd the second
+                                    # can be the z = Z + 1 ghost)
                                     z = zl + e
-                                    m = (z >= z0) & (z < z1) & own
+                                    m = (zl >= z0) & (zl < z1) & own
                                     if np.any(m):
                                         dst[q, np.broadcast_to(y, m.shape)[m], z[m]] = F[sl, wv, j, m, e]
                     load(s + 2 + D, (u - H + 1) % NR)
