@@ -542,21 +542,28 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     for (int i = 0; i < NKF; ++i)
 #pragma unroll
         for (int j = 0; j < RPW; ++j) KF[i][j] = u32x2_t{0u, 0u};
+    // a plane outside the tile's stream [pbase, lx1) is loaded with the
+    // scalar offset at the descriptor's record count, so every lane's offset
+    // is out of range and the load returns zeros (no per-lane select); the
+    // 32-bit sums stay below 2^32: lane offsets are below 2^31 - 128 and the
+    // record counts below 2^31 (a tile's planes span less than 2 GiB)
+    const int rec_f = (int)(span_f + 2u * kSlackBytes);
+    const int rec_c = (int)((uint32_t)(lx1 - pbase) * (uint32_t)plane_c);
     auto issue_load = [&](int p, auto slot_c) {
         constexpr int sl = decltype(slot_c)::value;
         const bool pok = p >= pbase && p < lx1;
-        const int sf = pok ? (p - pbase) * plane_f : 0, sc = pok ? (p - pbase) * plane_c : 0;
+        const int sf = pok ? (p - pbase) * plane_f : rec_f, sc = pok ? (p - pbase) * plane_c : rec_c;
 #pragma unroll
         for (int j = 0; j < RPW; ++j) {
-            F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, pok ? vf[j] : kOff, sf, 0);
+            F[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rF, vf[j], sf, 0);
             if constexpr (kFaces) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
-                    G[sl][j][k] = __builtin_amdgcn_raw_buffer_load_b64(rG[k], pok ? vf[j] : kOff, sf, 0);
+                    G[sl][j][k] = __builtin_amdgcn_raw_buffer_load_b64(rG[k], vf[j], sf, 0);
             } else if constexpr (kFaceCodes) {
-                KF[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rK, pok ? vf[j] : kOff, sf, 0);
+                KF[sl][j] = __builtin_amdgcn_raw_buffer_load_b64(rK, vf[j], sf, 0);
             } else {
-                K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, pok ? vc[j] : kOff, sc, 0);
+                K[sl][j] = __builtin_amdgcn_raw_buffer_load_b16(rC, vc[j], sc, 0);
             }
         }
     };
