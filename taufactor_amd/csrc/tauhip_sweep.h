@@ -393,11 +393,16 @@ __device__ __forceinline__ float* code_table() {
     }
 }
 
-template <int H, int KIND, int NW, int RPW>
+// AC: the active element of row 0 at step 0 of a block, (colour0 & 1) ^ 1 for
+// the launch's first colour (AA in the loop body); each value is its own
+// kernel, so that the register allocation of one colour's loop is not the
+// other's (the 512^3 Solver sweep ran 8 % slower in one of the two bodies of
+// a kernel holding both)
+template <int H, int KIND, int NW, int RPW, int AC>
 __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* __restrict__ D,
                                             const uint8_t* __restrict__ Cg,
                                             const float* __restrict__ hm, const Vol v, float w,
-                                            int colour0, int TX, int z0, int ty, int tx, bool nar,
+                                            int TX, int z0, int ty, int tx, bool nar,
                                             long long fsec = 0, int ncls = 0, bool wrap = false,
                                             bool wz = false) {
     using Sh = RegShape<H, NW, RPW>;
@@ -615,11 +620,10 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
     }
     // unclipped stream start: with TX and TY even, S0 + ey0 is odd, so the
     // active element of row j at step u of an unrolled block,
-    // (s + y + colour0) & 1, is A ^ ((u + j) & 1) with A = colour0 ^ 1 for
+    // (s + y + colour0) & 1, is AA ^ ((u + j) & 1) with AA = AC = colour0 ^ 1 for
     // every tile of the launch; the parity of S0 is that of H
     const int S0 = x0 - (H - 1);
     const int S1 = (x1 - 1) + (H - 1);
-    const int A = (colour0 & 1) ^ 1;
 
     // edge buffers: [h][parity][NW + 2][top, bottom][64 lanes]; wave w
     // publishes in slot w + 1, so the missing neighbours of the first and
@@ -884,13 +888,10 @@ __device__ __forceinline__ void rsweep_tile(const float* __restrict__ S, float* 
         }
         for (; s0 <= S1; s0 += NR) block(s0, std::true_type{});
     };
-    if (A)
-        body(ic<1>{});
-    else
-        body(ic<0>{});
+    body(ic<AC>{});
 }
 
-template <int H, int KIND, int NW, int RPW>
+template <int H, int KIND, int NW, int RPW, int AA>
 __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
     // XCD-aware tile numbering (grid is a multiple of 8)
     const int per = gridDim.x >> 3;
@@ -916,7 +917,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep(SweepArgs a) {
             ? reinterpret_cast<const uint8_t*>(static_cast<const float*>(a.codes) + off)
             : static_cast<const uint8_t*>(a.codes) + (size_t)b * sweep_code_volume(a.v.X, a.v.Y, a.v.Z);
     const float* hm = KIND == TF_KIND_ELECTRODE ? a.hm + (size_t)b * TF_ELECTRODE_TABLE : a.hm;
-    rsweep_tile<H, KIND, NW, RPW>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.colour0, a.TX,
+    rsweep_tile<H, KIND, NW, RPW, AA>(a.src + off, a.dst + off, codes, hm, a.v, a.w, a.TX,
                                   nar ? a.z0N : 1 + tz * RegShape<H, NW, RPW>::TZ, ty, tx, nar,
                                   a.fsec, a.ncls, a.periodic != 0, a.whole_z != 0);
 }
@@ -1005,14 +1006,17 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         if (lds + static_table_bytes(KIND) > 160 * 1024)
             return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H,
                         lds + static_table_bytes(KIND));
+        // one kernel per starting colour (rsweep_tile's AA)
+        const void* const fn[2] = {(const void*)k_sweep<H, KIND, NW, RPW, 0>,
+                                   (const void*)k_sweep<H, KIND, NW, RPW, 1>};
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
             // opt in to more than 64 KiB of dynamic LDS; if the runtime
             // refuses, the launch below reports the error
-            if (hipFuncSetAttribute((const void*)k_sweep<H, KIND, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(160 * 1024 - static_table_bytes(KIND))) != hipSuccess)
-                (void)hipGetLastError();
+            for (const void* f : fn)
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(160 * 1024 - static_table_bytes(KIND))) != hipSuccess)
+                    (void)hipGetLastError();
             attr_set = true;
         }
         a.nTy = (a.v.Y + Sh::TY - 1) / Sh::TY;
@@ -1029,9 +1033,11 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         }
         const long long yz = (long long)a.nTz * a.nTy + a.nTyN;
         if (a.TX <= 0) {
-            static int slots_cache[kMaxDevices] = {};
-            const int slots = resident_workgroups((const void*)k_sweep<H, KIND, NW, RPW>, NW * 64,
-                                                  lds, slots_cache);
+            // the fewer resident workgroups of the two colours' kernels, so
+            // that both colours of a solve tile alike
+            static int slots_cache[2][kMaxDevices] = {};
+            const int slots = std::min(resident_workgroups(fn[0], NW * 64, lds, slots_cache[0]),
+                                       resident_workgroups(fn[1], NW * 64, lds, slots_cache[1]));
             a.TX = balanced_tx(a.v.X, yz * B, H, Sh::NP, slots, true);
         }
         // the register-blocked kernel needs an even x tile (rsweep_tile),
@@ -1048,7 +1054,10 @@ static int launch_sweep(SweepArgs& a, int B, hipStream_t st) {
         a.total = (int)total;
         if (a.resolve_only) return 0;
         const int grid = (int)(((total + 7) / 8) * 8);
-        hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        if ((a.colour0 & 1) ^ 1)
+            hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW, 1>), dim3(grid), dim3(NW * 64), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sweep<H, KIND, NW, RPW, 0>), dim3(grid), dim3(NW * 64), lds, st, a);
         TF_LAUNCH_CHECK("k_sweep");
         return 0;
     }
@@ -1118,7 +1127,7 @@ struct BatchArgs {
 
 // Tile `logical` of the batch's tile list (every tile of every volume, in
 // volume order) for the active volumes; returns at once for the others.
-template <int H, int NW, int RPW>
+template <int H, int NW, int RPW, int AA>
 __device__ __forceinline__ void batch_tile(const BatchArgs& a, int logical) {
     using Sh = RegShape<H, NW, RPW>;
     const int vi = a.tile_vol[logical];
@@ -1150,16 +1159,16 @@ __device__ __forceinline__ void batch_tile(const BatchArgs& a, int logical) {
     }
     const float* src = (a.src_b ? a.fb : a.fa) + vol.offset;
     float* dst = (a.src_b ? a.fa : a.fb) + vol.offset;
-    rsweep_tile<H, TF_KIND_SOLVER, NW, RPW>(src, dst, a.codes + vol.code_offset, nullptr, v,
-                                            vol.omega, a.colour0, a.TX, z0, ty, tx, nar, 0, 0, false, wz);
+    rsweep_tile<H, TF_KIND_SOLVER, NW, RPW, AA>(src, dst, a.codes + vol.code_offset, nullptr, v,
+                                                vol.omega, a.TX, z0, ty, tx, nar, 0, 0, false, wz);
 }
 
-template <int H, int NW, int RPW>
+template <int H, int NW, int RPW, int AA>
 __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
     const int per = gridDim.x >> 3;
     const int logical = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (logical >= a.n_tiles) return;
-    batch_tile<H, NW, RPW>(a, logical);
+    batch_tile<H, NW, RPW, AA>(a, logical);
 }
 
 // The same pass as a work queue: a grid of at most the resident workgroups
@@ -1168,7 +1177,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch(BatchArgs a) {
 // a.queue[1 + a.pass], which k_batch_compact zeroed.  Every workgroup leaves
 // once the counter has passed the live count, so tiles of stopped volumes
 // are never dispatched and no workgroup waits for another.
-template <int H, int NW, int RPW>
+template <int H, int NW, int RPW, int AA>
 __global__ __launch_bounds__(NW * 64) void k_sweep_batch_queue(BatchArgs a) {
     __shared__ int s_next;
     const int n_live = a.queue[0];
@@ -1178,7 +1187,7 @@ __global__ __launch_bounds__(NW * 64) void k_sweep_batch_queue(BatchArgs a) {
         const int t = s_next;
         __syncthreads();  // every wave has read s_next before it is written again
         if (t >= n_live) break;
-        batch_tile<H, NW, RPW>(a, a.live[t]);
+        batch_tile<H, NW, RPW, AA>(a, a.live[t]);
     }
 }
 
@@ -1191,30 +1200,36 @@ static int launch_sweep_batch(BatchArgs& a, hipStream_t st) {
         const size_t lds = sweep_lds_bytes<H, NW, RPW>(TF_KIND_SOLVER, 0);
         if (lds + static_table_bytes(TF_KIND_SOLVER) > 160 * 1024)
             return fail(-1, "sweep shape %dx%d with n_half=%d needs %zu B of LDS", NW, RPW, H, lds);
+        const void* const fq[2] = {(const void*)k_sweep_batch_queue<H, NW, RPW, 0>,
+                                   (const void*)k_sweep_batch_queue<H, NW, RPW, 1>};
+        const void* const fb[2] = {(const void*)k_sweep_batch<H, NW, RPW, 0>,
+                                   (const void*)k_sweep_batch<H, NW, RPW, 1>};
         int grid = ((a.n_tiles + 7) / 8) * 8;
         if (a.queue) {
-            // the queue form: at most the resident workgroups
-            static int slots_cache[kMaxDevices] = {};
-            const int slots = resident_workgroups((const void*)k_sweep_batch_queue<H, NW, RPW>, NW * 64,
-                                                  lds, slots_cache);
+            // the queue form: at most the resident workgroups (the fewer of
+            // the two colours' kernels)
+            static int slots_cache[2][kMaxDevices] = {};
+            const int slots = std::min(resident_workgroups(fq[0], NW * 64, lds, slots_cache[0]),
+                                       resident_workgroups(fq[1], NW * 64, lds, slots_cache[1]));
             grid = slots > 0 ? std::min(a.n_tiles, slots) : std::min(a.n_tiles, 1024);
         }
         static bool attr_set = false;
         if (!attr_set && lds > 64 * 1024) {
-            if (hipFuncSetAttribute((const void*)k_sweep_batch_queue<H, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
-                (void)hipGetLastError();
-            if (hipFuncSetAttribute((const void*)k_sweep_batch<H, NW, RPW>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
-                (void)hipGetLastError();
+            for (const void* f : {fq[0], fq[1], fb[0], fb[1]})
+                if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(160 * 1024 - static_table_bytes(TF_KIND_SOLVER))) != hipSuccess)
+                    (void)hipGetLastError();
             attr_set = true;
         }
-        if (a.queue)
-            hipLaunchKernelGGL((k_sweep_batch_queue<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+        const bool aa = ((a.colour0 & 1) ^ 1) != 0;
+        if (a.queue && aa)
+            hipLaunchKernelGGL((k_sweep_batch_queue<H, NW, RPW, 1>), dim3(grid), dim3(NW * 64), lds, st, a);
+        else if (a.queue)
+            hipLaunchKernelGGL((k_sweep_batch_queue<H, NW, RPW, 0>), dim3(grid), dim3(NW * 64), lds, st, a);
+        else if (aa)
+            hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW, 1>), dim3(grid), dim3(NW * 64), lds, st, a);
         else
-            hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW>), dim3(grid), dim3(NW * 64), lds, st, a);
+            hipLaunchKernelGGL((k_sweep_batch<H, NW, RPW, 0>), dim3(grid), dim3(NW * 64), lds, st, a);
         TF_LAUNCH_CHECK("k_sweep_batch");
         return 0;
     }
