@@ -108,6 +108,13 @@ class BatchSolver:
             self._mask_offsets = mask_offsets
             self._setup_all()
             counts = self._count.cpu().numpy()
+            # the set-up of every volume (three launches each) recorded once as
+            # a graph that reset() replays: its kernels write the same buffers
+            # from the same inputs every time (no memset among them), and one
+            # replay costs far less than a launch per kernel from the host
+            self._setup_graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self._setup_graph):
+                self._setup_all()
         self.vol_x = []
         for v, (X, Y, Z) in enumerate(self.shapes):
             s0 = int(vols[v]["sum_offset"])
@@ -143,7 +150,7 @@ class BatchSolver:
     def reset(self):
         """Back to the initial fields (volumes resident on the device)."""
         with torch.cuda.device(self.device):
-            self._setup_all()
+            self._setup_graph.replay()
         self._plan = None
 
     @staticmethod
