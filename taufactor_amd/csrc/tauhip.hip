@@ -42,20 +42,21 @@ extern "C" const char* tf_last_error(void) { return g_err; }
 // mode 0: mask is 0/1 conductive flag, code = neighbour count
 // mode 1: mask is a class index, code = class (x ghosts copy the adjacent
 //         voxel's class, y/z ghosts class 0), conductive = cls_cond[class]
+// Grid (set_up_grid): x = (padded x plane, block of the plane), y = volume
+// b0 + blockIdx.y, so that the plane and the volume are uniform per block and
+// only the row is divided out per thread.
 template <int MODE>
 __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
                         const float* __restrict__ xinit, uint8_t* __restrict__ codes,
-                        float* __restrict__ fa, float* __restrict__ fb, Vol v, int B,
+                        float* __restrict__ fa, float* __restrict__ fb, Vol v, int b0, int nbx,
                         int periodic, int aniso) {
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)B * v.sB;
-    if (gid >= total) return;
-    int b = (int)(gid / v.sB);
-    int r = (int)(gid - (long long)b * v.sB);
-    int xp = r / v.sX;
-    int r2 = r - xp * v.sX;
-    int yp = r2 / v.sY;
-    int zp = r2 - yp * v.sY;
+    const int xp = (int)(blockIdx.x / (unsigned)nbx);
+    const int r2 = (int)(blockIdx.x - (unsigned)xp * (unsigned)nbx) * (int)blockDim.x + (int)threadIdx.x;
+    if (r2 >= v.sX) return;
+    const int b = b0 + (int)blockIdx.y;
+    const int r = xp * v.sX + r2;
+    const int yp = r2 / v.sY;
+    const int zp = r2 - yp * v.sY;
     const uint8_t* M = mask + (size_t)b * v.X * v.Y * v.Z;
     auto m_at = [&](int x, int y, int z) -> int {  // interior 0-based
         return (int)M[((size_t)x * v.Y + y) * v.Z + z];
@@ -65,69 +66,64 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
     float f;
     uint8_t code = 0;
     if (in) {
-        int x = xp - 1, y = yp - 1, z = zp - 1;
-        int m = m_at(x, y, z);
-        int cond = MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : (int)cls_cond[m];
+        const int x = xp - 1, y = yp - 1, z = zp - 1;
+        const uint8_t* P = M + ((size_t)x * v.Y + y) * v.Z + z;
+        const int m = P[0];
+        const int cond = MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : (int)cls_cond[m];
         // mask * linspace in float32 (taufactor/taufactor.py:291)
         f = (cond ? 1.0f : 0.0f) * xinit[x];
-        if (MODE == 2) {
-            if (cond) {
-                // conductive neighbours with x ghosts 2 (x = 0) and 0 (x = X+1),
-                // reactive neighbours with every ghost 0; y/z wrap round when
-                // periodic (taufactor/electrode.py:47-62, 133-151)
-                auto at = [&](int xx, int yy, int zz) -> int {
-                    if (periodic) {
-                        yy = (yy + v.Y) % v.Y;
-                        zz = (zz + v.Z) % v.Z;
-                    } else if (yy < 0 || yy >= v.Y || zz < 0 || zz >= v.Z) {
-                        return 0;
-                    }
-                    return m_at(xx, yy, zz);
-                };
-                int cn = 0, rn = 0;
-                const int nb[6][3] = {{x + 1, y, z}, {x - 1, y, z}, {x, y + 1, z},
-                                      {x, y - 1, z}, {x, y, z + 1}, {x, y, z - 1}};
-                for (int k = 0; k < 6; ++k) {
-                    const int xx = nb[k][0];
-                    if (xx < 0) { cn += 2; continue; }
-                    if (xx >= v.X) continue;
-                    const int mm = at(xx, nb[k][1], nb[k][2]);
-                    cn += mm == 1;
-                    rn += mm == 2;
+        if (MODE != 1) {
+            // the six neighbours, loaded unconditionally so that no load sits
+            // under a divergent branch: y/z offsets wrap round when periodic;
+            // a ghost neighbour's offset is 0 (the voxel itself) and its value
+            // is replaced below
+            const long long YZ = (long long)v.Y * v.Z;
+            const bool xq_in = x + 1 < v.X, xm_in = x >= 1;
+            const bool yq_in = y + 1 < v.Y, ym_in = y >= 1, zq_in = z + 1 < v.Z, zm_in = z >= 1;
+            const long long oxq = xq_in ? YZ : 0, oxm = xm_in ? -YZ : 0;
+            int oyq, oym, ozq, ozm;
+            if (periodic) {
+                oyq = (yq_in ? 1 : 1 - v.Y) * v.Z;
+                oym = (ym_in ? -1 : v.Y - 1) * v.Z;
+                ozq = zq_in ? 1 : 1 - v.Z;
+                ozm = zm_in ? -1 : v.Z - 1;
+            } else {
+                oyq = yq_in ? v.Z : 0;
+                oym = ym_in ? -v.Z : 0;
+                ozq = zq_in ? 1 : 0;
+                ozm = zm_in ? -1 : 0;
+            }
+            const int mxq = P[oxq], mxm = P[oxm], myq = P[oyq], mym = P[oym], mzq = P[ozq], mzm = P[ozm];
+            // y/z neighbours inside the volume (all of them when periodic)
+            const bool yq_ok = periodic || yq_in, ym_ok = periodic || ym_in;
+            const bool zq_ok = periodic || zq_in, zm_ok = periodic || zm_in;
+            if (MODE == 2) {
+                if (cond) {
+                    // conductive neighbours with x ghosts 2 (x = 0) and 0 (x = X+1),
+                    // reactive neighbours with every ghost 0; y/z wrap round when
+                    // periodic (taufactor/electrode.py:47-62, 133-151)
+                    const int cn = (xq_in ? (mxq == 1) : 0) + (xm_in ? (mxm == 1) : 2) + (yq_ok && myq == 1) +
+                                   (ym_ok && mym == 1) + (zq_ok && mzq == 1) + (zm_ok && mzm == 1);
+                    const int rn = (xq_in && mxq == 2) + (xm_in && mxm == 2) + (yq_ok && myq == 2) +
+                                   (ym_ok && mym == 2) + (zq_ok && mzq == 2) + (zm_ok && mzm == 2);
+                    code = (uint8_t)(1 + 7 * cn + rn);
                 }
-                code = (uint8_t)(1 + 7 * cn + rn);
-            }
-        } else if (MODE == 0 && aniso) {
-            if (cond) {
-                // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
-                // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
-                // order (taufactor/taufactor.py:459-471)
-                const int xm = (x >= 1) ? (m_at(x - 1, y, z) != 0) : 2;
-                const int xq = (x + 1 < v.X) ? (m_at(x + 1, y, z) != 0) : 2;
-                const int ym = (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
-                const int yq = (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
-                const int zm = (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
-                const int zq = (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
-                code = (uint8_t)(1 + 48 * xm + 16 * xq + 8 * ym + 4 * yq + 2 * zm + zq);
-            }
-        } else if (MODE == 0) {
-            if (cond) {
+            } else if (aniso) {
+                if (cond) {
+                    // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
+                    // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
+                    // order (taufactor/taufactor.py:459-471)
+                    const int xm = xm_in ? (mxm != 0) : 2;
+                    const int xq = xq_in ? (mxq != 0) : 2;
+                    const int ym = ym_in && mym != 0, yq = yq_in && myq != 0;
+                    const int zm = zm_in && mzm != 0, zq = zq_in && mzq != 0;
+                    code = (uint8_t)(1 + 48 * xm + 16 * xq + 8 * ym + 4 * yq + 2 * zm + zq);
+                }
+            } else if (cond) {
                 // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410),
                 // or y/z wrapped round (PeriodicSolver, taufactor/taufactor.py:493-499)
-                int nn = 0;
-                nn += (x + 1 < v.X) ? (m_at(x + 1, y, z) != 0) : 2;
-                nn += (x >= 1) ? (m_at(x - 1, y, z) != 0) : 2;
-                if (periodic) {
-                    nn += m_at(x, (y + 1) % v.Y, z) != 0;
-                    nn += m_at(x, (y + v.Y - 1) % v.Y, z) != 0;
-                    nn += m_at(x, y, (z + 1) % v.Z) != 0;
-                    nn += m_at(x, y, (z + v.Z - 1) % v.Z) != 0;
-                } else {
-                    nn += (y + 1 < v.Y) ? (m_at(x, y + 1, z) != 0) : 0;
-                    nn += (y >= 1) ? (m_at(x, y - 1, z) != 0) : 0;
-                    nn += (z + 1 < v.Z) ? (m_at(x, y, z + 1) != 0) : 0;
-                    nn += (z >= 1) ? (m_at(x, y, z - 1) != 0) : 0;
-                }
+                const int nn = (xq_in ? (mxq != 0) : 2) + (xm_in ? (mxm != 0) : 2) + (yq_ok && myq != 0) +
+                               (ym_ok && mym != 0) + (zq_ok && mzq != 0) + (zm_ok && mzm != 0);
                 code = (uint8_t)nn;  // 0 stays 0 = inf
             }
         } else {
@@ -164,26 +160,75 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
     fb[o] = f;
 }
 
-// conductive voxels per x layer; one block per (x, b)
-template <int MODE>
-__global__ void k_layer_count(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
-                              uint32_t* __restrict__ cnt, Vol v) {
-    int x = blockIdx.x, b = blockIdx.y;
-    const uint8_t* M = mask + ((size_t)b * v.X + x) * v.Y * v.Z;
-    int n = v.Y * v.Z;
-    uint32_t acc = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        int m = M[i];
-        acc += MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : cls_cond[m];
-    }
-    __shared__ uint32_t red[256];
-    red[threadIdx.x] = acc;
+constexpr int kSetupThreads = 256;
+constexpr int kCountThreads = 1024;
+constexpr int kMaxGridY = 65535;
+
+// Launch shape of k_setup: nbx blocks per padded x plane; the volumes go in
+// chunks of at most kMaxGridY along the grid's y.
+static int set_up_grid(const Vol& v, int* nbx, unsigned* gx) {
+    *nbx = (v.sX + kSetupThreads - 1) / kSetupThreads;
+    const long long blocks = (long long)*nbx * (v.X + 2);
+    if (blocks * kSetupThreads >= (1LL << 32)) return fail(-1, "volume too long in x for the set-up grid");
+    *gx = (unsigned)blocks;
+    return 0;
+}
+
+// Sum over the workgroup (kCountThreads lanes); the total is valid in
+// thread 0.  Integer counts: the order does not change the result.
+__device__ inline uint32_t workgroup_sum(uint32_t a) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += (uint32_t)__shfl_xor((int)a, o);
+    __shared__ uint32_t part[kCountThreads / 64];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
+    uint32_t s = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kCountThreads / 64; ++i) s += part[i];
+    return s;
+}
+
+// conductive flag of one mask byte, and the count over the four bytes of a
+// word: MODE 0 counts nonzero bytes (bit 7 of ((b & 0x7f) + 0x7f) | b is set
+// exactly when b != 0, and no byte carries into the next), MODE 2 bytes equal
+// to 1 (the nonzero test of b ^ 1), MODE 1 looks each class up
+template <int MODE>
+__device__ inline uint32_t conductive_byte(uint32_t m, const uint8_t* __restrict__ cls_cond) {
+    return MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : cls_cond[m];
+}
+template <int MODE>
+__device__ inline uint32_t conductive_word(uint32_t w, const uint8_t* __restrict__ cls_cond) {
+    if (MODE == 1)
+        return (uint32_t)cls_cond[w & 255u] + cls_cond[(w >> 8) & 255u] + cls_cond[(w >> 16) & 255u] +
+               cls_cond[w >> 24];
+    if (MODE == 2) w ^= 0x01010101u;
+    const uint32_t nz = __popc((((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w) & 0x80808080u);
+    return MODE == 2 ? 4u - nz : nz;
+}
+
+// conductive voxels per x layer; one workgroup per (x, b0 + blockIdx.y): the
+// layer's bytes up to a 16-byte boundary, then 16-byte loads, then the rest
+template <int MODE>
+__global__ void __launch_bounds__(kCountThreads)
+k_layer_count(const uint8_t* __restrict__ mask, const uint8_t* __restrict__ cls_cond,
+              uint32_t* __restrict__ cnt, Vol v, int b0) {
+    const int x = blockIdx.x, b = b0 + (int)blockIdx.y;
+    const long long n = (long long)v.Y * v.Z;
+    const uint8_t* M = mask + ((size_t)b * v.X + x) * (size_t)n;
+    const long long head = min(n, (long long)((16 - ((uintptr_t)M & 15)) & 15));
+    const long long nv = (n - head) / 16;
+    const long long tail = head + nv * 16;
+    uint32_t acc = 0;
+    if (threadIdx.x < head) acc += conductive_byte<MODE>(M[threadIdx.x], cls_cond);
+    const uint4* W = reinterpret_cast<const uint4*>(M + head);
+    for (long long i = threadIdx.x; i < nv; i += kCountThreads) {
+        const uint4 w = W[i];
+        acc += conductive_word<MODE>(w.x, cls_cond) + conductive_word<MODE>(w.y, cls_cond) +
+               conductive_word<MODE>(w.z, cls_cond) + conductive_word<MODE>(w.w, cls_cond);
     }
-    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = red[0];
+    if (tail + threadIdx.x < n) acc += conductive_byte<MODE>(M[tail + threadIdx.x], cls_cond);
+    const uint32_t total = workgroup_sum(acc);
+    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = total;
 }
 
 template <int MODE>
@@ -200,15 +245,20 @@ static int setup_common(const uint8_t* mask, const uint8_t* cls_cond, const floa
     if (fa == fb) return fail(-1, "field_a and field_b must be distinct buffers");
     Vol v = make_vol(X, Y, Z);
     hipStream_t st = (hipStream_t)stream;
-    long long total = (long long)B * v.sB;
-    int threads = 256;
-    long long blocks = (total + threads - 1) / threads;
-    hipLaunchKernelGGL(k_setup<MODE>, dim3((unsigned)blocks), dim3(threads), 0, st,
-                       mask, cls_cond, xinit, codes, fa, fb, v, B,
-                       (flags & TF_SETUP_PERIODIC_YZ) ? 1 : 0, (flags & TF_SETUP_ANISOTROPIC) ? 1 : 0);
-    TF_LAUNCH_CHECK("k_setup");
-    hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, B), dim3(256), 0, st, mask, cls_cond, cnt, v);
-    TF_LAUNCH_CHECK("k_layer_count");
+    int nbx;
+    unsigned gx;
+    if (int rc = set_up_grid(v, &nbx, &gx)) return rc;
+    if ((long long)X * kCountThreads >= (1LL << 32)) return fail(-1, "volume too long in x for the layer count");
+    for (int b0 = 0; b0 < B; b0 += kMaxGridY) {
+        const unsigned nb = (unsigned)min(B - b0, kMaxGridY);
+        hipLaunchKernelGGL(k_setup<MODE>, dim3(gx, nb), dim3(kSetupThreads), 0, st, mask, cls_cond, xinit,
+                           codes, fa, fb, v, b0, nbx, (flags & TF_SETUP_PERIODIC_YZ) ? 1 : 0,
+                           (flags & TF_SETUP_ANISOTROPIC) ? 1 : 0);
+        TF_LAUNCH_CHECK("k_setup");
+        hipLaunchKernelGGL(k_layer_count<MODE>, dim3(X, nb), dim3(kCountThreads), 0, st, mask, cls_cond, cnt,
+                           v, b0);
+        TF_LAUNCH_CHECK("k_layer_count");
+    }
     return 0;
 }
 
@@ -990,18 +1040,17 @@ extern "C" int tf_layer_sums_noalloc(const float* field, const void* codes, cons
 // kpad[b][x][y][z] = k of the voxel, x ghosts copy the adjacent voxel's k,
 // y/z ghosts 0 (the reference's diff_map padding, taufactor/taufactor.py:590-592);
 // both field buffers get the initial field, conductive = k > 0.
+// Same grid as k_setup.
 __global__ void k_setup_field(const float* __restrict__ k, const float* __restrict__ xinit,
                               float* __restrict__ kpad, float* __restrict__ fa,
-                              float* __restrict__ fb, Vol v, int B) {
-    long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    long long total = (long long)B * v.sB;
-    if (gid >= total) return;
-    int b = (int)(gid / v.sB);
-    int r = (int)(gid - (long long)b * v.sB);
-    int xp = r / v.sX;
-    int r2 = r - xp * v.sX;
-    int yp = r2 / v.sY;
-    int zp = r2 - yp * v.sY;
+                              float* __restrict__ fb, Vol v, int b0, int nbx) {
+    const int xp = (int)(blockIdx.x / (unsigned)nbx);
+    const int r2 = (int)(blockIdx.x - (unsigned)xp * (unsigned)nbx) * (int)blockDim.x + (int)threadIdx.x;
+    if (r2 >= v.sX) return;
+    const int b = b0 + (int)blockIdx.y;
+    const int r = xp * v.sX + r2;
+    const int yp = r2 / v.sY;
+    const int zp = r2 - yp * v.sY;
     const float* K = k + (size_t)b * v.X * v.Y * v.Z;
     const bool yz_in = (yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z);
     float f = 0.0f, kk = 0.0f;
@@ -1019,20 +1068,26 @@ __global__ void k_setup_field(const float* __restrict__ k, const float* __restri
     fb[o] = f;
 }
 
-__global__ void k_layer_count_field(const float* __restrict__ k, uint32_t* __restrict__ cnt, Vol v) {
-    int x = blockIdx.x, b = blockIdx.y;
-    const float* K = k + ((size_t)b * v.X + x) * v.Y * v.Z;
-    int n = v.Y * v.Z;
+// voxels with k > 0 per x layer, as k_layer_count (4-byte elements: up to
+// three before a 16-byte boundary)
+__global__ void __launch_bounds__(kCountThreads)
+k_layer_count_field(const float* __restrict__ k, uint32_t* __restrict__ cnt, Vol v, int b0) {
+    const int x = blockIdx.x, b = b0 + (int)blockIdx.y;
+    const long long n = (long long)v.Y * v.Z;
+    const float* K = k + ((size_t)b * v.X + x) * (size_t)n;
+    const long long head = min(n, (long long)(((16 - ((uintptr_t)K & 15)) & 15) / 4));
+    const long long nv = (n - head) / 4;
+    const long long tail = head + nv * 4;
     uint32_t acc = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += K[i] > 0.0f;
-    __shared__ uint32_t red[256];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s2 = blockDim.x / 2; s2 > 0; s2 >>= 1) {
-        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
-        __syncthreads();
+    if (threadIdx.x < head) acc += K[threadIdx.x] > 0.0f;
+    const float4* W = reinterpret_cast<const float4*>(K + head);
+    for (long long i = threadIdx.x; i < nv; i += kCountThreads) {
+        const float4 w = W[i];
+        acc += (uint32_t)(w.x > 0.0f) + (w.y > 0.0f) + (w.z > 0.0f) + (w.w > 0.0f);
     }
-    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = red[0];
+    if (tail + threadIdx.x < n) acc += K[tail + threadIdx.x] > 0.0f;
+    const uint32_t total = workgroup_sum(acc);
+    if (threadIdx.x == 0) cnt[(size_t)b * v.X + x] = total;
 }
 
 extern "C" int tf_setup_field(const float* k, const float* xinit, float* kpad, float* field_a,
@@ -1042,15 +1097,20 @@ extern "C" int tf_setup_field(const float* k, const float* xinit, float* kpad, f
     if (!k || !xinit || !kpad || !field_a || !field_b || !layer_count)
         return fail(-1, "null pointer argument");
     if (field_a == field_b) return fail(-1, "field_a and field_b must be distinct buffers");
-    if (X > 65535 || B > 65535) return fail(-1, "grid too large");
     Vol v = make_vol(X, Y, Z);
     hipStream_t st = (hipStream_t)stream;
-    long long total = (long long)B * v.sB;
-    hipLaunchKernelGGL(k_setup_field, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, k, xinit,
-                       kpad, field_a, field_b, v, B);
-    TF_LAUNCH_CHECK("k_setup_field");
-    hipLaunchKernelGGL(k_layer_count_field, dim3(X, B), dim3(256), 0, st, k, layer_count, v);
-    TF_LAUNCH_CHECK("k_layer_count_field");
+    int nbx;
+    unsigned gx;
+    if (int rc = set_up_grid(v, &nbx, &gx)) return rc;
+    if ((long long)X * kCountThreads >= (1LL << 32)) return fail(-1, "volume too long in x for the layer count");
+    for (int b0 = 0; b0 < B; b0 += kMaxGridY) {
+        const unsigned nb = (unsigned)min(B - b0, kMaxGridY);
+        hipLaunchKernelGGL(k_setup_field, dim3(gx, nb), dim3(kSetupThreads), 0, st, k, xinit, kpad, field_a,
+                           field_b, v, b0, nbx);
+        TF_LAUNCH_CHECK("k_setup_field");
+        hipLaunchKernelGGL(k_layer_count_field, dim3(X, nb), dim3(kCountThreads), 0, st, k, layer_count, v, b0);
+        TF_LAUNCH_CHECK("k_layer_count_field");
+    }
     return 0;
 }
 
