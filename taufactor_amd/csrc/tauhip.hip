@@ -590,32 +590,36 @@ extern "C" int tf_sweep_launch_config(int kind, int n_classes, int B, int X, int
 // offset from the neighbourhood key; ghosts get pair 0.
 __global__ void k_face_codes(const uint8_t* __restrict__ C, const uint8_t* __restrict__ fmap,
                              const uint16_t* __restrict__ keys, uint32_t* __restrict__ out, Vol v,
-                             int B) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long total = (long long)B * v.sB;
-    if (gid >= total) return;
-    const int b = (int)(gid / v.sB);
-    const int r = (int)(gid - (long long)b * v.sB);
-    const int xp = r / v.sX, r2 = r - xp * v.sX;
+                             int b0, int nbx) {
+    // grid as k_setup; the seven class loads (and then the seven face-class
+    // lookups) are unconditional, from offsets that fall back to the voxel
+    // itself where a neighbour lies outside the padded volume or is not needed
+    const int xp = (int)(blockIdx.x / (unsigned)nbx);
+    const int r2 = (int)(blockIdx.x - (unsigned)xp * (unsigned)nbx) * (int)blockDim.x + (int)threadIdx.x;
+    if (r2 >= v.sX) return;
+    const int b = b0 + (int)blockIdx.y;
+    const int r = xp * v.sX + r2;
     const int yp = r2 / v.sY, zp = r2 - yp * v.sY;
-    const uint8_t* V = C + (size_t)b * v.sB;
-    auto fc = [&](int o) { return (uint32_t)fmap[V[o]]; };
-    const uint32_t c = fc(r);
-    const uint32_t fx = xp <= v.X ? 4u * (4u * c + fc(r + v.sX)) : 0u;
-    const uint32_t fy = yp <= v.Y ? 4u * (4u * c + fc(r + v.sY)) : 0u;
-    const uint32_t fz = zp <= v.Z ? 4u * (4u * c + fc(r + 1)) : 0u;
-    uint32_t pair = 0;
-    if (xp >= 1 && xp <= v.X && yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z) {
-        uint32_t key = c;
-        key = key * 4 + fc(r - v.sX);
-        key = key * 4 + fc(r + v.sX);
-        key = key * 4 + fc(r - v.sY);
-        key = key * 4 + fc(r + v.sY);
-        key = key * 4 + fc(r - 1);
-        key = key * 4 + fc(r + 1);
-        key = key * 4 + (xp == 1 ? 1u : 0u) + (xp == v.X ? 2u : 0u);
-        pair = keys[key];
-    }
+    const uint8_t* P = C + (size_t)b * v.sB + r;
+    const bool inner = xp >= 1 && xp <= v.X && yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z;
+    const int oxq = xp <= v.X ? v.sX : 0, oyq = yp <= v.Y ? v.sY : 0, ozq = zp <= v.Z ? 1 : 0;
+    const int oxm = inner ? -v.sX : 0, oym = inner ? -v.sY : 0, ozm = inner ? -1 : 0;
+    const uint32_t c = fmap[P[0]];
+    const uint32_t cxq = fmap[P[oxq]], cyq = fmap[P[oyq]], czq = fmap[P[ozq]];
+    const uint32_t cxm = fmap[P[oxm]], cym = fmap[P[oym]], czm = fmap[P[ozm]];
+    const uint32_t fx = xp <= v.X ? 4u * (4u * c + cxq) : 0u;
+    const uint32_t fy = yp <= v.Y ? 4u * (4u * c + cyq) : 0u;
+    const uint32_t fz = zp <= v.Z ? 4u * (4u * c + czq) : 0u;
+    uint32_t key = c;
+    key = key * 4 + cxm;
+    key = key * 4 + cxq;
+    key = key * 4 + cym;
+    key = key * 4 + cyq;
+    key = key * 4 + czm;
+    key = key * 4 + czq;
+    key = key * 4 + (xp == 1 ? 1u : 0u) + (xp == v.X ? 2u : 0u);
+    const uint32_t pk = keys[inner ? key : 0u];
+    const uint32_t pair = inner ? pk : 0u;
     out[(size_t)b * v.sB + r] = fx | (fy << 6) | (fz << 12) | (pair << 18);
 }
 
@@ -626,11 +630,14 @@ extern "C" int tf_multiphase_face_codes(const uint8_t* codes, const uint8_t* fac
     if (!codes || !face_class || !key_table || !face_codes) return fail(-1, "null pointer argument");
     if (n_classes < 1 || n_classes > 256) return fail(-1, "1..256 classes expected (got %d)", n_classes);
     Vol v = make_vol(X, Y, Z);
-    const long long total = (long long)B * v.sB;
-    const int threads = 256;
-    hipLaunchKernelGGL(k_face_codes, dim3((unsigned)((total + threads - 1) / threads)), dim3(threads), 0,
-                       (hipStream_t)stream, codes, face_class, key_table, face_codes, v, B);
-    TF_LAUNCH_CHECK("k_face_codes");
+    int nbx;
+    unsigned gx;
+    if (int rc = set_up_grid(v, &nbx, &gx)) return rc;
+    for (int b0 = 0; b0 < B; b0 += kMaxGridY) {
+        hipLaunchKernelGGL(k_face_codes, dim3(gx, (unsigned)min(B - b0, kMaxGridY)), dim3(kSetupThreads), 0,
+                           (hipStream_t)stream, codes, face_class, key_table, face_codes, v, b0, nbx);
+        TF_LAUNCH_CHECK("k_face_codes");
+    }
     return 0;
 }
 
