@@ -1060,15 +1060,15 @@ __global__ void k_setup_field(const float* __restrict__ k, const float* __restri
     const int zp = r2 - yp * v.sY;
     const float* K = k + (size_t)b * v.X * v.Y * v.Z;
     const bool yz_in = (yp >= 1 && yp <= v.Y && zp >= 1 && zp <= v.Z);
-    float f = 0.0f, kk = 0.0f;
-    if (yz_in && xp >= 1 && xp <= v.X) {
-        kk = K[((size_t)(xp - 1) * v.Y + (yp - 1)) * v.Z + (zp - 1)];
-        f = (kk > 0.0f ? 1.0f : 0.0f) * xinit[xp - 1];
-    } else if (yz_in) {
-        const int x = xp == 0 ? 0 : v.X - 1;
-        kk = K[((size_t)x * v.Y + (yp - 1)) * v.Z + (zp - 1)];
-        f = xp == 0 ? -1.0f : 1.0f;
-    }
+    const bool x_in = xp >= 1 && xp <= v.X;
+    // one unconditional load: the voxel, the adjacent layer's for an x ghost,
+    // a clamped one (unused) for a y/z ghost
+    const int xs = xp == 0 ? 0 : (x_in ? xp - 1 : v.X - 1);
+    const int ys = min(max(yp - 1, 0), v.Y - 1), zs = min(max(zp - 1, 0), v.Z - 1);
+    const float kl = K[((size_t)xs * v.Y + ys) * v.Z + zs];
+    const float xi = xinit[xs];
+    const float kk = yz_in ? kl : 0.0f;
+    const float f = !yz_in ? 0.0f : x_in ? (kl > 0.0f ? 1.0f : 0.0f) * xi : (xp == 0 ? -1.0f : 1.0f);
     const size_t o = (size_t)b * v.sB + r;
     kpad[o] = kk;
     fa[o] = f;
@@ -1128,16 +1128,21 @@ extern "C" int tf_setup_field(const float* k, const float* xinit, float* kpad, f
 // (doubling is exact, the product is rounded once), so the face of voxels u
 // and v is what the update of either computes (MultiPhaseSolver's harmonic
 // mean, taufactor/taufactor.py:577-583).
-__global__ void k_field_faces(const float* __restrict__ kpad, float* __restrict__ faces, Vol v, int B,
-                              long long fsec) {
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= (long long)B * v.sB) return;
-    const int r = (int)(gid % v.sB);
-    const int xp = r / v.sX, yp = (r - xp * v.sX) / v.sY, zp = r - xp * v.sX - yp * v.sY;
-    const float kc = kpad[gid];
-    faces[gid] = xp <= v.X ? hmean(kc, kpad[gid + v.sX]) : 0.0f;
-    faces[fsec + gid] = yp <= v.Y ? hmean(kc, kpad[gid + v.sY]) : 0.0f;
-    faces[2 * fsec + gid] = zp <= v.Z ? hmean(kc, kpad[gid + 1]) : 0.0f;
+__global__ void k_field_faces(const float* __restrict__ kpad, float* __restrict__ faces, Vol v, int b0,
+                              int nbx, long long fsec) {
+    // grid as k_setup; the neighbour loads are unconditional (offset 0, the
+    // voxel itself, where the neighbour lies outside the padded volume)
+    const int xp = (int)(blockIdx.x / (unsigned)nbx);
+    const int r2 = (int)(blockIdx.x - (unsigned)xp * (unsigned)nbx) * (int)blockDim.x + (int)threadIdx.x;
+    if (r2 >= v.sX) return;
+    const int yp = r2 / v.sY, zp = r2 - yp * v.sY;
+    const size_t o = (size_t)(b0 + (int)blockIdx.y) * v.sB + (size_t)xp * v.sX + r2;
+    const bool xq = xp <= v.X, yq = yp <= v.Y, zq = zp <= v.Z;
+    const float* K = kpad + o;
+    const float kc = K[0], kx = K[xq ? v.sX : 0], ky = K[yq ? v.sY : 0], kz = K[zq ? 1 : 0];
+    faces[o] = xq ? hmean(kc, kx) : 0.0f;
+    faces[fsec + o] = yq ? hmean(kc, ky) : 0.0f;
+    faces[2 * fsec + o] = zq ? hmean(kc, kz) : 0.0f;
 }
 
 extern "C" int tf_field_faces(const float* kpad, float* faces, int B, int X, int Y, int Z,
@@ -1146,9 +1151,14 @@ extern "C" int tf_field_faces(const float* kpad, float* faces, int B, int X, int
     if (!kpad || !faces) return fail(-1, "null pointer argument");
     const Vol v = make_vol(X, Y, Z);
     const long long total = (long long)B * v.sB;
-    hipLaunchKernelGGL(k_field_faces, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, kpad, faces, v, B, total);
-    TF_LAUNCH_CHECK("k_field_faces");
+    int nbx;
+    unsigned gx;
+    if (int rc = set_up_grid(v, &nbx, &gx)) return rc;
+    for (int b0 = 0; b0 < B; b0 += kMaxGridY) {
+        hipLaunchKernelGGL(k_field_faces, dim3(gx, (unsigned)min(B - b0, kMaxGridY)), dim3(kSetupThreads), 0,
+                           (hipStream_t)stream, kpad, faces, v, b0, nbx, total);
+        TF_LAUNCH_CHECK("k_field_faces");
+    }
     return 0;
 }
 
