@@ -53,6 +53,12 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert st < 0 and b"flags" in lib.tf_last_error()
     st = lib.tf_wrap_ghosts_from(None, 1, 1, 4, 4, 4, None)
     assert st < 0 and b"null pointer" in lib.tf_last_error()
+    # set-up grids: 256 lanes per block and a block per 256 positions of each
+    # padded x plane; 20M planes of 3 x 3 exceed the 2^32 lanes of a launch
+    st = lib.tf_setup_solver(1, 2, 3, 4, 5, 6, 1, 20_000_000, 1, 1, 0, None)
+    assert st < 0 and b"too long in x" in lib.tf_last_error()
+    st = lib.tf_field_faces(1, 2, 1, 20_000_000, 1, 1, None)
+    assert st < 0 and b"too long in x" in lib.tf_last_error()
 
 
 def test_spans_x_host_entry_point():
