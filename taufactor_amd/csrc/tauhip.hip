@@ -39,9 +39,13 @@ extern "C" const char* tf_last_error(void) { return g_err; }
 // set-up kernels
 // ---------------------------------------------------------------------------
 // One thread per padded voxel.  Writes codes and both field buffers.
-// mode 0: mask is 0/1 conductive flag, code = neighbour count
+// mode 0: mask is 0/1 conductive flag, code = neighbour count (anisotropic:
+//         the six neighbour states)
 // mode 1: mask is a class index, code = class (x ghosts copy the adjacent
-//         voxel's class, y/z ghosts class 0), conductive = cls_cond[class]
+//         voxel's class, y/z ghosts class 0 or, periodic, the opposite
+//         layer's), conductive = cls_cond[class]
+// mode 2: mask is 0 / 1 conductive / 2 reactive, code = 1 + 7 x conductive
+//         neighbours + reactive neighbours for conductive voxels
 // Grid (set_up_grid): x = (padded x plane, block of the plane), y = volume
 // b0 + blockIdx.y, so that the plane and the volume are uniform per block and
 // only the row is divided out per thread.
@@ -65,69 +69,90 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
     bool in = yz_in && xp >= 1 && xp <= v.X;
     float f;
     uint8_t code = 0;
-    if (in) {
+    if constexpr (MODE == 1) {
+        // class codes: one unconditional load of the class the padded voxel
+        // takes: its own; the adjacent layer's for an x ghost (diff_map[:,0] =
+        // diff_map[:,1], taufactor/taufactor.py:591-592); with periodic y/z the
+        // opposite interior layer's for a y/z ghost, after the x copy
+        // (PeriodicMultiPhaseSolver, taufactor/taufactor.py:633-638); closed
+        // y/z ghosts are class 0 (their clamped load goes unused)
+        const int xs = xp == 0 ? 0 : (xp == v.X + 1 ? v.X - 1 : xp - 1);
+        int ys, zs;
+        if (periodic) {
+            ys = (yp == 0 ? v.Y : (yp == v.Y + 1 ? 1 : yp)) - 1;
+            zs = (zp == 0 ? v.Z : (zp == v.Z + 1 ? 1 : zp)) - 1;
+        } else {
+            ys = min(max(yp - 1, 0), v.Y - 1);
+            zs = min(max(zp - 1, 0), v.Z - 1);
+        }
+        const int m = m_at(xs, ys, zs);
+        const int cond = cls_cond[m];
+        const float xi = xinit[xs];
+        code = (periodic || yz_in) ? (uint8_t)m : (uint8_t)0;
+        // mask * linspace in float32 (taufactor/taufactor.py:291); ghosts: x
+        // planes hold 2*bc on their y/z interior, everything else 0
+        // (SORSolver._pad, taufactor/taufactor.py:234-237)
+        f = in ? (cond ? 1.0f : 0.0f) * xi
+               : (yz_in && xp == 0) ? -1.0f : (yz_in && xp == v.X + 1) ? 1.0f : 0.0f;
+    } else if (in) {
         const int x = xp - 1, y = yp - 1, z = zp - 1;
         const uint8_t* P = M + ((size_t)x * v.Y + y) * v.Z + z;
         const int m = P[0];
-        const int cond = MODE == 0 ? (m != 0) : MODE == 2 ? (m == 1) : (int)cls_cond[m];
+        const int cond = MODE == 0 ? (m != 0) : (m == 1);
         // mask * linspace in float32 (taufactor/taufactor.py:291)
         f = (cond ? 1.0f : 0.0f) * xinit[x];
-        if (MODE != 1) {
-            // the six neighbours, loaded unconditionally so that no load sits
-            // under a divergent branch: y/z offsets wrap round when periodic;
-            // a ghost neighbour's offset is 0 (the voxel itself) and its value
-            // is replaced below
-            const long long YZ = (long long)v.Y * v.Z;
-            const bool xq_in = x + 1 < v.X, xm_in = x >= 1;
-            const bool yq_in = y + 1 < v.Y, ym_in = y >= 1, zq_in = z + 1 < v.Z, zm_in = z >= 1;
-            const long long oxq = xq_in ? YZ : 0, oxm = xm_in ? -YZ : 0;
-            int oyq, oym, ozq, ozm;
-            if (periodic) {
-                oyq = (yq_in ? 1 : 1 - v.Y) * v.Z;
-                oym = (ym_in ? -1 : v.Y - 1) * v.Z;
-                ozq = zq_in ? 1 : 1 - v.Z;
-                ozm = zm_in ? -1 : v.Z - 1;
-            } else {
-                oyq = yq_in ? v.Z : 0;
-                oym = ym_in ? -v.Z : 0;
-                ozq = zq_in ? 1 : 0;
-                ozm = zm_in ? -1 : 0;
-            }
-            const int mxq = P[oxq], mxm = P[oxm], myq = P[oyq], mym = P[oym], mzq = P[ozq], mzm = P[ozm];
-            // y/z neighbours inside the volume (all of them when periodic)
-            const bool yq_ok = periodic || yq_in, ym_ok = periodic || ym_in;
-            const bool zq_ok = periodic || zq_in, zm_ok = periodic || zm_in;
-            if (MODE == 2) {
-                if (cond) {
-                    // conductive neighbours with x ghosts 2 (x = 0) and 0 (x = X+1),
-                    // reactive neighbours with every ghost 0; y/z wrap round when
-                    // periodic (taufactor/electrode.py:47-62, 133-151)
-                    const int cn = (xq_in ? (mxq == 1) : 0) + (xm_in ? (mxm == 1) : 2) + (yq_ok && myq == 1) +
-                                   (ym_ok && mym == 1) + (zq_ok && mzq == 1) + (zm_ok && mzm == 1);
-                    const int rn = (xq_in && mxq == 2) + (xm_in && mxm == 2) + (yq_ok && myq == 2) +
-                                   (ym_ok && mym == 2) + (zq_ok && mzq == 2) + (zm_ok && mzm == 2);
-                    code = (uint8_t)(1 + 7 * cn + rn);
-                }
-            } else if (aniso) {
-                if (cond) {
-                    // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
-                    // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
-                    // order (taufactor/taufactor.py:459-471)
-                    const int xm = xm_in ? (mxm != 0) : 2;
-                    const int xq = xq_in ? (mxq != 0) : 2;
-                    const int ym = ym_in && mym != 0, yq = yq_in && myq != 0;
-                    const int zm = zm_in && mzm != 0, zq = zq_in && mzq != 0;
-                    code = (uint8_t)(1 + 48 * xm + 16 * xq + 8 * ym + 4 * yq + 2 * zm + zq);
-                }
-            } else if (cond) {
-                // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410),
-                // or y/z wrapped round (PeriodicSolver, taufactor/taufactor.py:493-499)
-                const int nn = (xq_in ? (mxq != 0) : 2) + (xm_in ? (mxm != 0) : 2) + (yq_ok && myq != 0) +
-                               (ym_ok && mym != 0) + (zq_ok && mzq != 0) + (zm_ok && mzm != 0);
-                code = (uint8_t)nn;  // 0 stays 0 = inf
-            }
+        // the six neighbours, loaded unconditionally so that no load sits
+        // under a divergent branch: y/z offsets wrap round when periodic;
+        // a ghost neighbour's offset is 0 (the voxel itself) and its value
+        // is replaced below
+        const long long YZ = (long long)v.Y * v.Z;
+        const bool xq_in = x + 1 < v.X, xm_in = x >= 1;
+        const bool yq_in = y + 1 < v.Y, ym_in = y >= 1, zq_in = z + 1 < v.Z, zm_in = z >= 1;
+        const long long oxq = xq_in ? YZ : 0, oxm = xm_in ? -YZ : 0;
+        int oyq, oym, ozq, ozm;
+        if (periodic) {
+            oyq = (yq_in ? 1 : 1 - v.Y) * v.Z;
+            oym = (ym_in ? -1 : v.Y - 1) * v.Z;
+            ozq = zq_in ? 1 : 1 - v.Z;
+            ozm = zm_in ? -1 : v.Z - 1;
         } else {
-            code = (uint8_t)m;
+            oyq = yq_in ? v.Z : 0;
+            oym = ym_in ? -v.Z : 0;
+            ozq = zq_in ? 1 : 0;
+            ozm = zm_in ? -1 : 0;
+        }
+        const int mxq = P[oxq], mxm = P[oxm], myq = P[oyq], mym = P[oym], mzq = P[ozq], mzm = P[ozm];
+        // y/z neighbours inside the volume (all of them when periodic)
+        const bool yq_ok = periodic || yq_in, ym_ok = periodic || ym_in;
+        const bool zq_ok = periodic || zq_in, zm_ok = periodic || zm_in;
+        if (MODE == 2) {
+            if (cond) {
+                // conductive neighbours with x ghosts 2 (x = 0) and 0 (x = X+1),
+                // reactive neighbours with every ghost 0; y/z wrap round when
+                // periodic (taufactor/electrode.py:47-62, 133-151)
+                const int cn = (xq_in ? (mxq == 1) : 0) + (xm_in ? (mxm == 1) : 2) + (yq_ok && myq == 1) +
+                               (ym_ok && mym == 1) + (zq_ok && mzq == 1) + (zm_ok && mzm == 1);
+                const int rn = (xq_in && mxq == 2) + (xm_in && mxm == 2) + (yq_ok && myq == 2) +
+                               (ym_ok && mym == 2) + (zq_ok && mzq == 2) + (zm_ok && mzm == 2);
+                code = (uint8_t)(1 + 7 * cn + rn);
+            }
+        } else if (aniso) {
+            if (cond) {
+                // neighbour states x-,x+ (0/1, 2 for the x ghost), y-,y+,z-,z+
+                // (0/1, ghosts 0): the terms AnisotropicSolver sums, in its
+                // order (taufactor/taufactor.py:459-471)
+                const int xm = xm_in ? (mxm != 0) : 2;
+                const int xq = xq_in ? (mxq != 0) : 2;
+                const int ym = ym_in && mym != 0, yq = yq_in && myq != 0;
+                const int zm = zm_in && mzm != 0, zq = zq_in && mzq != 0;
+                code = (uint8_t)(1 + 48 * xm + 16 * xq + 8 * ym + 4 * yq + 2 * zm + zq);
+            }
+        } else if (cond) {
+            // padded mask: x ghosts 2, y/z ghosts 0 (taufactor/taufactor.py:404-410),
+            // or y/z wrapped round (PeriodicSolver, taufactor/taufactor.py:493-499)
+            const int nn = (xq_in ? (mxq != 0) : 2) + (xm_in ? (mxm != 0) : 2) + (yq_ok && myq != 0) +
+                           (ym_ok && mym != 0) + (zq_ok && mzq != 0) + (zm_ok && mzm != 0);
+            code = (uint8_t)nn;  // 0 stays 0 = inf
         }
     } else {
         // ghost: x planes hold 2*bc on their y/z interior, everything else 0
@@ -139,19 +164,6 @@ __global__ void k_setup(const uint8_t* __restrict__ mask, const uint8_t* __restr
         } else {
             if (yz_in && xp == 0) f = -1.0f;
             if (yz_in && xp == v.X + 1) f = 1.0f;
-        }
-        if (MODE == 1 && yz_in && (xp == 0 || xp == v.X + 1)) {
-            // diff_map[:,0] = diff_map[:,1] (taufactor/taufactor.py:591-592)
-            int x = xp == 0 ? 0 : v.X - 1;
-            code = (uint8_t)m_at(x, yp - 1, zp - 1);
-        }
-        if (MODE == 1 && periodic) {
-            // y/z ghosts copy the opposite interior layer, after the x copy
-            // (PeriodicMultiPhaseSolver, taufactor/taufactor.py:633-638)
-            const int y = (yp == 0 ? v.Y : (yp == v.Y + 1 ? 1 : yp)) - 1;
-            const int z = (zp == 0 ? v.Z : (zp == v.Z + 1 ? 1 : zp)) - 1;
-            const int x = xp == 0 ? 0 : (xp == v.X + 1 ? v.X - 1 : xp - 1);
-            code = (uint8_t)m_at(x, y, z);
         }
     }
     size_t o = (size_t)b * v.sB + r;
