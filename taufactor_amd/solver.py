@@ -802,7 +802,8 @@ class MultiPhaseSolver(ThroughTransportSolver):
     kind = _lib.KIND_MULTIPHASE
     # classes (labels + the zero ghost class) of the class-code sweep and of
     # grad_D_eff (LDS tables); the face-code sweep, the per-voxel field sweep
-    # (closed boundaries) and the layer sums take up to MAX_LABELS labels
+    # (closed boundaries) and the layer sums take up to MAX_LABELS classes:
+    # one per label, or one per distinct diffusivity beyond MAX_LABELS labels
     MAX_CLASSES = 64
     MAX_LABELS = 255
     # sweep through per-voxel face codes when the diffusivities allow it
@@ -847,11 +848,21 @@ class MultiPhaseSolver(ThroughTransportSolver):
         d_label = np.array([np.float32(self.Ds.get(lbl.item(), 0.0)) for lbl in labels], dtype=np.float32)
         # classes: 0 = the zero-diffusivity ghost class, then one per label
         # present in the image (labels stay separate so that gradients can be
-        # reported per label)
-        if len(labels) > self.MAX_LABELS:
-            raise ValueError(f"at most {self.MAX_LABELS} distinct labels are supported")
-        d_cls = np.concatenate([np.zeros(1, dtype=np.float32), d_label]).astype(np.float32)
-        cls_of_label = np.arange(1, len(labels) + 1, dtype=np.uint8)
+        # reported per label); with more labels than the one-byte class codes
+        # hold, one class per distinct diffusivity: the sweep, the layer sums
+        # and tau see only the diffusivities (diff_map,
+        # taufactor/taufactor.py:586-589), the per-label gradient is refused
+        self._grouped = len(labels) > self.MAX_LABELS
+        if self._grouped:
+            d_values, cls_idx = np.unique(d_label, return_inverse=True)
+            if len(d_values) > self.MAX_LABELS:
+                raise ValueError(f"more than {self.MAX_LABELS} distinct labels are supported when their "
+                                 f"diffusivities take at most {self.MAX_LABELS} values")
+            d_cls = np.concatenate([np.zeros(1, dtype=np.float32), d_values]).astype(np.float32)
+            cls_of_label = (cls_idx.ravel() + 1).astype(np.uint8)
+        else:
+            d_cls = np.concatenate([np.zeros(1, dtype=np.float32), d_label]).astype(np.float32)
+            cls_of_label = np.arange(1, len(labels) + 1, dtype=np.uint8)
         # face conductance table: hm(a,b) = 2*a*b/(a+b), 0 where a+b == 0
         # (MultiPhaseSolver._harmonic_mean, taufactor/taufactor.py:577-583)
         a = d_cls[:, None]
@@ -961,7 +972,7 @@ class MultiPhaseSolver(ThroughTransportSolver):
         reference solves under torch.no_grad).  The derivative of the discrete
         fixed point: see include/tauhip.h, tf_grad_classes."""
         B, X, Y, Z = self.batch_size, self.Nx, self.Ny, self.Nz
-        if self._n_classes > self.MAX_CLASSES:
+        if self._n_classes > self.MAX_CLASSES or self._grouped:
             raise ValueError(f"grad_D_eff supports at most {self.MAX_CLASSES - 1} distinct labels")
         out = torch.empty((B, X, self._n_classes), dtype=torch.float64, device=self.device)
         with torch.cuda.device(self.device):

@@ -13,7 +13,9 @@ diffusivities fit the face tables (at most four distinct values with 0).
   still equals the oracle;
 * a full solve stops at the oracle's iteration with its tau;
 * more than 63 labels with more than four diffusivities sweep as the
-  per-voxel field kind and still equal the oracle."""
+  per-voxel field kind and still equal the oracle;
+* more than 255 labels take one class per distinct diffusivity and still
+  equal the oracle."""
 import numpy as np
 import pytest
 import torch
@@ -211,3 +213,27 @@ def test_many_labels_with_many_diffusivities(H):
         assert float(s.tau[0]) == float(np.float32(o.tau[0]))
         with pytest.raises(ValueError):
             tau.PeriodicMultiPhaseSolver(rng.choice(labels, size=(6, 8, 10)), dict(many))
+
+
+@pytest.mark.parametrize("n_values", [3, 70])
+def test_more_labels_than_class_codes(n_values):
+    """More than 255 labels (the one-byte class codes' limit) take one class
+    per distinct diffusivity: with three diffusivities the face-code sweep
+    serves them, with seventy the per-voxel field sweep.  A full solve stops
+    at the oracle's iteration with its field and tau; the per-label gradient
+    is refused."""
+    rng = np.random.default_rng(23 + n_values)
+    labels = np.arange(1000, 1300)
+    vals = [0.0, 1.0, 0.1] if n_values == 3 else [float(np.float32(0.05 * (i + 1))) for i in range(70)]
+    Ds = {int(lbl): vals[i % len(vals)] for i, lbl in enumerate(labels)}
+    img = rng.choice(labels, size=(16, 14, 18))
+    s = tau.MultiPhaseSolver(img, dict(Ds))
+    assert s._grouped and s._n_classes == len(set(vals)) + 1
+    assert (s._faces is not None) if n_values == 3 else (s._field_faces is not None)
+    s.solve(verbose=False, conv_crit=1e-3)
+    o = OracleMultiPhaseSolver(img, dict(Ds)).solve(conv_crit=1e-3)
+    assert s.iter == o.iter
+    assert np.array_equal(s.field.cpu().numpy(), o.field.numpy())
+    assert float(s.tau[0]) == float(np.float32(o.tau[0]))
+    with pytest.raises(ValueError):
+        s.grad_D_eff()
