@@ -1304,6 +1304,49 @@ __global__ __launch_bounds__(256) void k_grad_classes(const float* __restrict__ 
     }
 }
 
+// The same sums for up to NC classes with 16 waves per layer: wave w takes
+// the rows y = w, w + 16, ..., its lanes z = lane, lane + 64, ...; each lane
+// adds its voxels' gradients into one register per class, and the per-class
+// totals are reduced once at the end (wave shuffle tree, then the 16 waves in
+// order), so the result does not depend on scheduling.
+template <int NC>
+__global__ __launch_bounds__(1024) void k_grad_classes_reg(const float* __restrict__ F,
+                                                           const uint8_t* __restrict__ C,
+                                                           const float* __restrict__ dcls, int ncls,
+                                                           Vol v, double scale,
+                                                           double* __restrict__ out) {
+    constexpr int kWaves = 16;
+    const int x = blockIdx.x, b = blockIdx.y;
+    const float* V = F + (size_t)b * v.sB;
+    const uint8_t* Cv = C + (size_t)b * v.sB;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto kat = [&](int i) { return (double)dcls[Cv[i]]; };
+    double acc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+    for (int y = wave; y < v.Y; y += kWaves) {
+        for (int z = lane; z < v.Z; z += 64) {
+            const int o = (x + 1) * v.sX + (y + 1) * v.sY + (z + 1);
+            const int cl = Cv[o];
+            const double g = voxel_grad(V, o, v, x, kat);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[c] += cl == c ? g : 0.0;
+        }
+    }
+    __shared__ double part[kWaves][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const double s = wave_sum(acc[c]);
+        if (lane == 0) part[wave][c] = s;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncls) {
+        double s2 = 0.0;
+        for (int w = 0; w < kWaves; ++w) s2 += part[w][threadIdx.x];
+        out[((size_t)b * v.X + x) * ncls + threadIdx.x] = s2 * scale;
+    }
+}
+
 extern "C" int tf_grad_classes(const float* field, const uint8_t* codes, const float* d_cls,
                                int n_classes, int B, int X, int Y, int Z, double* grad_layers,
                                void* stream) {
@@ -1313,8 +1356,18 @@ extern "C" int tf_grad_classes(const float* field, const uint8_t* codes, const f
     if (X > 65535 || B > 65535) return fail(-1, "grid too large for gradient");
     Vol v = make_vol(X, Y, Z);
     const double scale = (double)X / ((double)Y * (double)Z);
-    hipLaunchKernelGGL(k_grad_classes, dim3(X, B), dim3(256), 0, (hipStream_t)stream, field, codes,
-                       d_cls, n_classes, v, scale, grad_layers);
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 grid(X, B);
+    if (n_classes <= 4) {
+        hipLaunchKernelGGL(k_grad_classes_reg<4>, grid, dim3(1024), 0, st, field, codes, d_cls, n_classes, v,
+                           scale, grad_layers);
+    } else if (n_classes <= 8) {
+        hipLaunchKernelGGL(k_grad_classes_reg<8>, grid, dim3(1024), 0, st, field, codes, d_cls, n_classes, v,
+                           scale, grad_layers);
+    } else {
+        hipLaunchKernelGGL(k_grad_classes, grid, dim3(256), 0, st, field, codes, d_cls, n_classes, v, scale,
+                           grad_layers);
+    }
     TF_LAUNCH_CHECK("k_grad_classes");
     return 0;
 }
