@@ -1236,11 +1236,14 @@ __device__ __forceinline__ double voxel_grad(const float* __restrict__ V, int o,
     return g;
 }
 
+// one thread per voxel; a block is 4 rows of 64 z (full waves wherever Z is
+// a multiple of 64)
 __global__ void k_grad_field(const float* __restrict__ F, const float* __restrict__ K, Vol v, int B,
                              double scale, float* __restrict__ grad) {
-    const int z = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = blockIdx.y;
+    const int z = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int bx = blockIdx.z;
+    if (y >= v.Y) return;
     const int b = bx / v.X, x = bx - b * v.X;
     if (z >= v.Z) return;
     const float* V = F + (size_t)b * v.sB;
@@ -1254,10 +1257,10 @@ extern "C" int tf_grad_field(const float* field, const float* kpad, int B, int X
                              float* grad, void* stream) {
     if (int rc = check_dims(B, X, Y, Z)) return rc;
     if (!field || !kpad || !grad) return fail(-1, "null pointer argument");
-    if ((long long)B * X > 65535 || Y > 65535) return fail(-1, "grid too large for gradient");
+    if ((long long)B * X > 65535 || (Y + 3) / 4 > 65535) return fail(-1, "grid too large for gradient");
     Vol v = make_vol(X, Y, Z);
     const double scale = (double)X / ((double)Y * (double)Z);
-    hipLaunchKernelGGL(k_grad_field, dim3((Z + 255) / 256, Y, B * X), dim3(256), 0,
+    hipLaunchKernelGGL(k_grad_field, dim3((Z + 63) / 64, (Y + 3) / 4, B * X), dim3(256), 0,
                        (hipStream_t)stream, field, kpad, v, B, scale, grad);
     TF_LAUNCH_CHECK("k_grad_field");
     return 0;
