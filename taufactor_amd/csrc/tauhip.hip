@@ -1188,48 +1188,53 @@ extern "C" int tf_field_faces(const float* kpad, float* faces, int B, int X, int
 //            conductive neighbours and Dirichlet faces (weights all 2):
 //            sum 2 (c_u - c*)^2 + sum 2 (c* - bc)^2
 // Evaluated in float64 from the float32 field.
+// Every value is loaded first (the six neighbours of an interior voxel lie in
+// the padded volume; an x ghost's values go unused), so that no load waits
+// behind a branch on an earlier one; the terms and their order are those of
+// the formula above.
 template <typename KFUN>
 __device__ __forceinline__ double voxel_grad(const float* __restrict__ V, int o, const Vol v, int x,
                                              KFUN kat) {
     const int offs[6] = {v.sX, -v.sX, v.sY, -v.sY, 1, -1};
     const double kv = kat(o);
     const double cv = V[o];
+    double ku[6], cu[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        ku[i] = kat(o + offs[i]);
+        cu[i] = V[o + offs[i]];
+    }
+    // the x neighbours of the end layers are the Dirichlet faces, added below
+    bool face[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) face[i] = !((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) && ku[i] > 0.0;
     double g = 0.0;
     if (kv > 0.0) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;  // Dirichlet faces below
-            const double ku = kat(o + offs[i]);
-            if (ku <= 0.0) continue;
-            const double d = (double)V[o + offs[i]] - cv;
-            const double s = kv + ku;
-            g += d * d * (2.0 * ku * ku) / (s * s);
+            if (!face[i]) continue;
+            const double d = cu[i] - cv;
+            const double s = kv + ku[i];
+            g += d * d * (2.0 * ku[i] * ku[i]) / (s * s);
         }
         if (x == 0) g += 2.0 * (cv + 0.5) * (cv + 0.5);
         if (x == v.X - 1) g += 2.0 * (cv - 0.5) * (cv - 0.5);
     } else {
         double sum = 0.0;
         int n = 0;
-        double cu[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-            cu[i] = 0.0;
-            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;
-            if (kat(o + offs[i]) > 0.0) {
-                cu[i] = V[o + offs[i]];
-                sum += cu[i];
-                ++n;
-            }
+            if (!face[i]) continue;
+            sum += cu[i];
+            ++n;
         }
         if (x == 0) { sum += -0.5; ++n; }
         if (x == v.X - 1) { sum += 0.5; ++n; }
         if (n == 0) return 0.0;
         const double cs = sum / n;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            if ((i == 0 && x == v.X - 1) || (i == 1 && x == 0)) continue;
-            if (kat(o + offs[i]) > 0.0) g += 2.0 * (cu[i] - cs) * (cu[i] - cs);
-        }
+        for (int i = 0; i < 6; ++i)
+            if (face[i]) g += 2.0 * (cu[i] - cs) * (cu[i] - cs);
         if (x == 0) g += 2.0 * (cs + 0.5) * (cs + 0.5);
         if (x == v.X - 1) g += 2.0 * (cs - 0.5) * (cs - 0.5);
     }
