@@ -83,6 +83,8 @@ class SORSolver(ABC):
 
         self._fields = None  # two padded float32 buffers (ping-pong)
         self._cur = 0
+        self._spare = None  # third buffer: more sweeps ahead of a check
+        self._checked = None  # (buffers, index) of the checked field
         self._launch_events = None  # list -> (start, end, n_launches) per _advance
         self._codes = None
         self._sweep_codes = None
@@ -92,6 +94,9 @@ class SORSolver(ABC):
         with torch.cuda.device(self.device):
             self._setup_device()
             self._run_setup()
+            if self._zero_yz_ghosts and self._fields[0].numel() <= self.SPARE_MAX_VOXELS:
+                self._spare = _lib.slack_buffer(tuple(self._fields[0].shape), torch.float32, self.device)
+                self._spare.copy_(self._fields[0])
             # per-layer float64 sums, flux (B x (X-1), or B x X for the
             # electrode kind) then concentration (B x X), in one buffer so
             # that a check copies them to the host in one transfer
@@ -127,8 +132,11 @@ class SORSolver(ABC):
         host image: the device-resident input is packed again."""
         with torch.cuda.device(self.device):
             self._run_setup()
+            if self._spare is not None:
+                self._spare.copy_(self._fields[0])
         self._cur = 0
         self._ahead = 0
+        self._checked = None
         self._sums_ready = None
         self.converged = False
         self.old_tau = 0
@@ -164,7 +172,7 @@ class SORSolver(ABC):
         if self._zero_yz_ghosts and value.dim() == 4 and (
                 torch.any(value[:, :, [0, -1], :] != 0) or torch.any(value[:, :, :, [0, -1]] != 0)):
             raise ValueError("the y/z ghost layers of the field must hold 0 (closed boundaries)")
-        for f in self._fields:
+        for f in self._fields + ([self._spare] if self._spare is not None else []):
             f.copy_(value)
 
     @property
@@ -308,23 +316,50 @@ class SORSolver(ABC):
         decision: its first launch (the periodic solvers override)."""
         return n_next
 
+    # volumes of at most this many padded voxels keep a third field buffer,
+    # so that up to MAX_AHEAD sweeps run while the host decides on a check
+    # (its decision outlasts one short sweep); larger volumes run one ahead
+    SPARE_MAX_VOXELS = 1 << 23
+    MAX_AHEAD = 3
+
     def _launch_ahead(self, n_next: int) -> None:
-        """Launch the first sweep of the next block of n_next iterations while
-        the host decides on the check just enqueued.  It reads the checked
-        field and writes the other buffer, so if the solve stops there,
-        _drop_ahead returns to the checked field unchanged."""
+        """Launch the first sweeps of the next block of n_next iterations
+        while the host decides on the check just enqueued.  The first reads
+        the checked field and writes the other buffer; with a third buffer the
+        second writes that one, which takes the checked field's place in the
+        pair, so no sweep writes the checked field and, if the solve stops
+        there, _drop_ahead returns to it unchanged."""
         room = self._ahead_room(n_next)
         if room <= 0:
             return
         per, tile, codes, hm = self._sweep_args()
-        h = min(per, room)
-        self._launch(h, self.iter, tile, codes, hm)
-        self._ahead = h
+        launches = 1
+        if self._spare is not None:
+            launches = self.MAX_AHEAD
+            self._checked = (list(self._fields), self._cur)
+        done = 0
+        for i in range(launches):
+            h = min(per, room - done)
+            if h <= 0:
+                break
+            if i == 1:
+                k = 1 - self._cur  # the checked field, after the first launch
+                self._fields[k], self._spare = self._spare, self._fields[k]
+            self._launch(h, self.iter + done, tile, codes, hm)
+            done += h
+        self._ahead = done
 
     def _drop_ahead(self) -> None:
         if self._ahead:
-            self._cur = 1 - self._cur
+            if self._checked is not None:
+                fields, cur = self._checked
+                pool = self._fields + [self._spare]
+                self._spare = next(t for t in pool if all(t is not f for f in fields))
+                self._fields, self._cur = list(fields), cur
+            else:
+                self._cur = 1 - self._cur
             self._ahead = 0
+        self._checked = None
 
     def _enqueue_layer_sums(self) -> None:
         """Layer sums of the current field and their copy to pinned host
