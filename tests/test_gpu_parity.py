@@ -220,7 +220,7 @@ def test_solver_z_tiling_boundaries(H, Z, Y):
 
 @pytest.mark.parametrize("which", ["Solver", "PeriodicSolver", "PeriodicSolver_odd", "MultiPhaseSolver"])
 def test_checks_with_sweep_launched_ahead(which):
-    """At each convergence check the first sweep of the next block is
+    """At each convergence check the first sweeps of the next block are
     launched before the host's decision and dropped if the solve stops.
     Checks that do not stop (conv_crit 0), blocks cut by iter_limit, a solve
     resumed after a check, and a check that stops must all leave the
@@ -242,6 +242,9 @@ def test_checks_with_sweep_launched_ahead(which):
                      (tau.PeriodicSolver, OraclePeriodicSolver))
         make, oracle = (lambda: cls(img)), (lambda: ocls(img))
     s = make()
+    # small closed volumes keep a third buffer and run three sweeps ahead;
+    # the periodic solvers run one
+    assert (s._spare is None) == which.startswith("Periodic")
     s.half_sweeps_per_pass = 6
     s.solve(iter_limit=250, verbose=False, conv_crit=0.0)   # checks at 100, 200; stops at 250
     assert s.iter == 250 and not s.converged
