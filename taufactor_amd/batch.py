@@ -108,13 +108,16 @@ class BatchSolver:
             self._mask_offsets = mask_offsets
             self._setup_all()
             counts = self._count.cpu().numpy()
-            # the set-up of every volume (three launches each) recorded once as
-            # a graph that reset() replays: its kernels write the same buffers
-            # from the same inputs every time (no memset among them), and one
-            # replay costs far less than a launch per kernel from the host
-            self._setup_graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(self._setup_graph):
-                self._setup_all()
+            # a solve changes only the two field buffers, which the set-up
+            # fills alike: reset() copies this initial field back into both
+            # (a copy at HBM rate instead of three set-up launches per volume).
+            # Whole 16-byte quads: the copy may take up to three floats of the
+            # zero slack after the field, which this copy holds as zeros too
+            n4 = (off + 3) // 4 * 4
+            self._reset_bytes = 4 * n4
+            self._field_init = torch.empty(n4, dtype=torch.float32, device=self.device)
+            _lib.check(self._lib.tf_hbm_copy(_lib.ptr(self._fa), _lib.ptr(self._field_init), self._reset_bytes,
+                                             self._stream()), "tf_hbm_copy")
         self.vol_x = []
         for v, (X, Y, Z) in enumerate(self.shapes):
             s0 = int(vols[v]["sum_offset"])
@@ -150,7 +153,9 @@ class BatchSolver:
     def reset(self):
         """Back to the initial fields (volumes resident on the device)."""
         with torch.cuda.device(self.device):
-            self._setup_graph.replay()
+            for dst in (self._fa, self._fb):
+                _lib.check(self._lib.tf_hbm_copy(_lib.ptr(self._field_init), _lib.ptr(dst), self._reset_bytes,
+                                                 self._stream()), "tf_hbm_copy")
         self._plan = None
 
     @staticmethod

@@ -97,9 +97,10 @@ def test_work_queue_equals_full_dispatch(use_graph):
 
 
 def test_solve_twice_gives_the_same_result():
-    """Later solves start from reset(), which replays the set-up recorded as a
-    graph at construction: every replay leaves the buffers the eager set-up
-    left, and every solve gives the first one's result."""
+    """Later solves start from reset(), which copies the initial field saved
+    at construction into both field buffers: every reset leaves the buffers
+    the set-up left (codes and counts untouched by the solves), and every
+    solve gives the first one's result."""
     vols = _volumes(seed=6, n=5)
     b = tau.BatchSolver(vols)
     n = b._n_padded
