@@ -7,7 +7,7 @@ Default run (one GPU, one JSON line on stdout):
             quantile) solved to the reference's default criterion (conv_crit
             1e-2, checks every 100 iterations).  One step = one complete
             solve from the initial field with the volume already resident in
-            HBM (Solver.reset() re-packs it on the device).
+            HBM (Solver.reset() copies the initial field back on the device).
             value = GVoxel-updates/s = voxels x (iterations/2) x steps / time
             (one voxel-update = one voxel through both colours).
   configs   the other workloads of the benchmark table, each measured the

@@ -94,9 +94,15 @@ class SORSolver(ABC):
         with torch.cuda.device(self.device):
             self._setup_device()
             self._run_setup()
+            # a solve changes only the field buffers, which the set-up fills
+            # alike: reset() copies this initial field back into them (at the
+            # copy rate) instead of running the set-up again
+            shape = tuple(self._fields[0].shape)
+            self._field_init = _lib.slack_buffer(shape, torch.float32, self.device)
+            self._copy_field(self._fields[0], self._field_init)
             if self._zero_yz_ghosts and self._fields[0].numel() <= self.SPARE_MAX_VOXELS:
-                self._spare = _lib.slack_buffer(tuple(self._fields[0].shape), torch.float32, self.device)
-                self._spare.copy_(self._fields[0])
+                self._spare = _lib.slack_buffer(shape, torch.float32, self.device)
+                self._copy_field(self._field_init, self._spare)
             # per-layer float64 sums, flux (B x (X-1), or B x X for the
             # electrode kind) then concentration (B x X), in one buffer so
             # that a check copies them to the host in one transfer
@@ -127,13 +133,20 @@ class SORSolver(ABC):
     def _run_setup(self) -> None:
         """Write codes and both field buffers from the device-resident input."""
 
+    def _copy_field(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        """dst := src for two field buffers (tf_hbm_copy in whole 16-byte
+        quads: up to three floats of the zero slack after the field go
+        along, zeros on both sides)."""
+        n = (src.numel() + 3) // 4 * 16
+        _lib.check(self._lib.tf_hbm_copy(_lib.ptr(src), _lib.ptr(dst), n, self._stream()), "tf_hbm_copy")
+
     def reset(self) -> None:
         """Return to the initial field and iteration 0 without touching the
-        host image: the device-resident input is packed again."""
+        host image: the initial field the set-up wrote is copied back (the
+        codes and tables the set-up built do not change during a solve)."""
         with torch.cuda.device(self.device):
-            self._run_setup()
-            if self._spare is not None:
-                self._spare.copy_(self._fields[0])
+            for f in self._fields + ([self._spare] if self._spare is not None else []):
+                self._copy_field(self._field_init, f)
         self._cur = 0
         self._ahead = 0
         self._checked = None

@@ -400,3 +400,35 @@ def test_hbm_copy_kernel():
         torch.cuda.synchronize()
         assert torch.equal(dst[:n], src) and not torch.any(dst[n:])
     assert lib.tf_hbm_copy(_lib.ptr(src), _lib.ptr(dst), 20, stream) != 0
+
+
+@pytest.mark.parametrize("which", ["Solver", "Solver_large", "MultiPhaseSolver", "FieldSolver", "PeriodicSolver",
+                                   "PeriodicSolver_odd", "ElectrodeSolver"])
+def test_reset_returns_to_the_initial_field(which):
+    """reset() copies the initial field the set-up wrote back into the field
+    buffers (and into the third buffer of small closed volumes): the field
+    equals a fresh solver's bit for bit, and a second solve repeats the
+    first.  Solver_large has no third buffer."""
+    shape = {"PeriodicSolver_odd": (8, 9, 11), "Solver_large": (40, 300, 720)}.get(which, (16, 14, 18))
+    if which == "MultiPhaseSolver":
+        img = _labels(shape, n=3, seed=31)
+        make = lambda: tau.MultiPhaseSolver(img, {0: 0.0, 1: 1.0, 2: 0.2})  # noqa: E731
+    elif which == "FieldSolver":
+        k = np.random.default_rng(31).uniform(0.2, 2.0, size=shape).astype(np.float32)
+        make = lambda: tau.FieldSolver(k)  # noqa: E731
+    elif which == "ElectrodeSolver":
+        img = np.random.default_rng(31).choice([0, 1, 5], size=shape, p=[0.3, 0.55, 0.15])
+        make = lambda: tau.ElectrodeSolver(img)  # noqa: E731
+    else:
+        img = _binary(shape, p=0.7, seed=31)
+        cls = tau.PeriodicSolver if which.startswith("Periodic") else tau.Solver
+        make = lambda: cls(img)  # noqa: E731
+    s = make()
+    assert (s._spare is not None) == (which in ("Solver", "MultiPhaseSolver", "FieldSolver", "ElectrodeSolver"))
+    fresh = s.field.cpu().numpy().copy()
+    s.solve(iter_limit=300, verbose=False)
+    first = (s.iter, s.field.cpu().numpy().copy())
+    s.reset()
+    assert s.iter == 0 and np.array_equal(s.field.cpu().numpy(), fresh)
+    s.solve(iter_limit=300, verbose=False)
+    assert s.iter == first[0] and np.array_equal(s.field.cpu().numpy(), first[1])
